@@ -1,0 +1,181 @@
+/* zcsmi.h — C ABI of libzcsmi.so, zcmir's SMI hot path on AMD MI355X (gfx950).
+ *
+ * Drop-in boundary for the reference's native library (src/exports.zig, src/native_api.zig) as
+ * bound by python/zcmir/_native.py:79-133. Plain pointers and sizes; no C++ or torch types.
+ * Every call returns 0 (or a size / count) on success and -1 on failure; zc_error() returns the
+ * reason (exports.zig:1-16). An engine is one HIP device and stream; calls on one engine are
+ * serialized by the caller (python/zcmir/api.py:29). Pose H: row-major 3x3 f64, moving → fixed
+ * pixels, 1-based (python/zcmir/__init__.py:18-20). Outputs are caller-allocated with explicit
+ * lengths; a too-small buffer fails with MapTooSmall / BufferTooSmall (engine.zig:802,
+ * exports.zig:269).
+ */
+#ifndef ZCSMI_H
+#define ZCSMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* engine.zig:32-53 ShiftResult (88 bytes) */
+typedef struct {
+  uint32_t n, cw, ch, canvas_w, canvas_h;
+  int32_t ox, oy;
+  uint32_t peak_index;
+  double dx, dy, peak, zero, corr_area_px;
+  uint32_t exact, calibrated, n_y, _pad;
+} zc_shift_result;
+
+/* engine.zig:55-73 RsResult (96 bytes) */
+typedef struct {
+  uint32_t n, n_th, n_lam, peak_index;
+  double dlam, r0, r1, cx, cy, peak, zero, dth, dsg;
+  uint32_t symmetric, calibrated;
+} zc_rs_result;
+
+/* engine.zig:76-85 GradResult (616 bytes) */
+typedef struct {
+  double score, fwd, inv, n;
+  double grad[8];
+  double hess[64];
+  uint32_t has_hess, nk;
+} zc_grad_result;
+
+/* search.zig:30-41 Result */
+typedef struct {
+  double H[9];
+  double score, prev;
+  uint32_t improved, count_a, count_b, _pad;
+} zc_search_result;
+
+/* climb.zig:7-12 Result */
+typedef struct {
+  double H[9];
+  double score;
+  uint32_t iterations, moved;
+} zc_climb_result;
+
+/* ── lifetime (native_api.zig:15-52) ── */
+/* native_api.zig:20 zc_version */
+const char* zc_version(void);
+/* native_api.zig:26 zc_create: `dev` is a HIP device index as text ("0"); any other text
+ * (the reference's wgpu-native path) selects device 0 */
+void* zc_create(const char* dev);
+/* native_api.zig:41 */
+size_t zc_create_error(char* buf, size_t cap);
+/* native_api.zig:47 */
+void zc_destroy(void* e);
+/* exports.zig:22: the device name and gfx arch */
+size_t zc_adapter(void* e, char* buf, size_t cap);
+/* exports.zig:30 */
+size_t zc_error(void* e, char* buf, size_t cap);
+
+/* ── settings (exports.zig:35, 42; settings.zig:14-199) ── */
+int32_t zc_configure(void* e, const char* json, size_t len);
+size_t zc_settings(void* e, char* buf, size_t cap);
+
+/* ── images and features ── */
+/* exports.zig:50: a work image, no preprocessing */
+int32_t zc_set_image(void* e, uint32_t side, const float* data, uint32_t w, uint32_t h);
+/* exports.zig:57; preprocess=1 is accepted only with clahe, band and invert off */
+int32_t zc_set_image_gray(void* e, uint32_t side, const float* data, uint32_t w, uint32_t h, uint32_t preprocess);
+/* exports.zig:63 (always preprocessed: same restriction) */
+int32_t zc_set_image_rgba(void* e, uint32_t side, const uint8_t* rgba, uint32_t w, uint32_t h);
+/* exports.zig:238 */
+void zc_image_size(void* e, uint32_t side, uint32_t out[2]);
+/* exports.zig:320 */
+int32_t zc_work_image(void* e, uint32_t side, float* out, size_t len);
+/* exports.zig:264: 4 x h x w plane-major whitened features */
+int32_t zc_features(void* e, uint32_t side, float* out, size_t len);
+/* exports.zig:226 */
+void zc_swap(void* e);
+
+/* ── scores and maps ── */
+/* exports.zig:276: the 45 overlap sums of moving onto fixed at H */
+int32_t zc_moments(void* e, const double H[9], double out[45]);
+/* exports.zig:417: [mean, forward, inverse] (inverse NaN when not symmetric) */
+int32_t zc_score(void* e, const double H[9], double out[3]);
+/* exports.zig:424 (returns -1 "Unsupported" in this build) */
+int32_t zc_gradient(void* e, const double H[9], uint32_t hess, zc_grad_result* out);
+/* exports.zig:436: flags 1 exact | 2 calibrated, cap the FFT length cap (0: 1024), no
+ * overlap floor; map (nullable) receives n_y x n floats */
+int32_t zc_shift_map(void* e, const double H[9], uint32_t flags, uint32_t cap, zc_shift_result* out, float* map,
+                     size_t map_len);
+/* exports.zig:442: the settings' score, resolution (map_res) and overlap floor (min_overlap) */
+int32_t zc_shift_map_set(void* e, const double H[9], zc_shift_result* out, float* map, size_t map_len);
+/* exports.zig:449 (returns -1 "Unsupported" in this build) */
+int32_t zc_rs_map(void* e, const double H[9], const double c[2], uint32_t symmetric, zc_rs_result* out, float* map,
+                  float* fwd, float* inv, size_t len);
+/* exports.zig:79: one 2-D FFT of `planes` n x n complex planes in place (interleaved f32;
+ * inverse scaled by 1/n^2); n a power of two 32..4096; four_step ignored */
+int32_t zc_fft2(void* e, float* data, uint32_t n, uint32_t planes, uint32_t inverse, uint32_t four_step);
+
+/* ── pose, search, climb ── */
+/* exports.zig:388 */
+void zc_set_pose(void* e, const double H[9]);
+/* exports.zig:392 */
+void zc_get_pose(void* e, double H[9]);
+/* exports.zig:344 (returns -1 "Unsupported" in this build) */
+int32_t zc_search(void* e, zc_search_result* out);
+/* exports.zig:405 (returns -1 "Unsupported" in this build) */
+int32_t zc_climb(void* e, zc_climb_result* out);
+/* exports.zig:221 */
+void zc_cancel(void* e);
+
+/* ── events, profiling, counters ── */
+/* exports.zig:412: JSON-lines events (none are queued by this build) */
+size_t zc_events(void* e, char* buf, size_t cap);
+/* exports.zig:210: per-kernel timing on (1) / off (0); resets the report */
+int32_t zc_profile(void* e, uint32_t on);
+/* exports.zig:215: lines "kernel launches total_ms mean_ms" */
+size_t zc_profile_report(void* e, char* buf, size_t cap);
+/* exports.zig:110: dispatches, submits, reads, waits since the last call */
+void zc_gpu_counts(void* e, uint32_t out[4]);
+/* exports.zig:497 */
+void zc_result_sizes(uint32_t out[4]);
+
+/* ── this library's own hooks (no counterpart in the reference) ── */
+/* the rank transform of a side: rank (w*h f32, nullable), hist[257], cdf[258], stats[2] =
+ * (mean, spread) — for bit-exact checks of the rank transform */
+int32_t zc_rank(void* e, uint32_t side, float* rank, size_t len, uint32_t* hist, uint32_t* cdf, float* stats);
+/* `iters` shift maps back to back (flags, cap as zc_shift_map); mean ms per map, HIP events */
+int32_t zc_bench_shift_map(void* e, const double H[9], uint32_t flags, uint32_t cap, uint32_t iters,
+                           double* ms_per_map);
+
+/* ── reference entry points outside the SMI hot path (exports.zig:95-138, 232, 245, 284-376,
+ *    382, 397, 430, 458-489): exported so the reference's full ctypes symbol list binds; each
+ *    fails with zc_error "Unsupported" (or is a no-op tuning hook) ── */
+int32_t zc_compile_wgsl(void* e, const char* code, size_t code_len, const char* entry, size_t entry_len, char* msg,
+                        size_t cap);
+int32_t zc_pg_info(void* e, double out[5]);
+int32_t zc_pg_maps(void* e, uint32_t side, float* out, size_t len);
+int32_t zc_pg_descriptors(void* e, uint32_t side, float* out, size_t len);
+int32_t zc_bench_moments(void* e, uint32_t n_poses, uint32_t iters);
+int32_t zc_match_scores(void* e, float* out, size_t cap);
+int32_t zc_detect(void* e, void* out);
+int32_t zc_match(void* e, void* out);
+int32_t zc_auto(void* e, void* out);
+int32_t zc_keypoints(void* e, uint32_t side, float* out, size_t cap_points);
+int32_t zc_keypoint_frames(void* e, uint32_t side, float* out, size_t cap_points);
+int32_t zc_matches(void* e, uint32_t* out, size_t cap);
+int32_t zc_set_ffd(void* e, const float* cps, size_t n);
+size_t zc_get_ffd(void* e, float* cps, size_t cap);
+uint32_t zc_fit_check(void* e, const double H[9]);
+int32_t zc_ffd_gradient(void* e, const double H[9], double* score, double* grad, size_t len);
+int32_t zc_canvas(void* e, const double H[9], uint32_t inverse, void* out);
+int32_t zc_overlay(void* e, const double H[9], uint32_t inverse, void* out, float* moving, float* fixed, size_t len);
+int32_t zc_tile_heat(void* e, const double H[9], uint32_t grid, uint32_t min_n, double* out, float* dest, float* src,
+                     size_t len);
+int64_t zc_export_warp(void* e, const char* stem, size_t stem_len, void* out, size_t cap);
+int32_t zc_displacement(void* e, float* out, size_t len);
+void zc_tune_parts(uint32_t px_per_thread, uint32_t fill_wg, uint32_t tg_px, uint32_t tg_cap);
+void zc_fft_tune(uint32_t rows, uint32_t cols);
+void zc_fft_tune_wg(uint32_t rows, uint32_t cols, uint32_t balanced);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZCSMI_H */

@@ -1,0 +1,150 @@
+"""GPU parity of the SMI hot path against the float64 oracle, through the C ABI.
+
+Bars: rank transform (histogram, CDF, bins, rank values, mean/spread) bit-exact; maps within
+max |Δ| / max |map| <= 1e-4 on lags clear of the overlap floor, with the same argmax lag;
+FFT within 2e-6 of the largest output magnitude (float32 Stockham against float64 pocketfft).
+"""
+import numpy as np
+import pytest
+
+from oracle import smi_ref as O
+from tests._maps import compare
+from zcmir_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+MAP_TOL = 1e-4
+
+
+def _pair(size, shift, seed=0, quantize=False):
+    return synth.shift_pair(size, shift, seed=seed, quantize=quantize)
+
+
+@pytest.mark.parametrize("seed,size,half", [(0, (256, 256), False), (1, (97, 131), False), (2, (64, 200), True),
+                                             (3, (300, 257), True)])
+def test_rank_bit_exact(reg_factory, seed, size, half):
+    rng = np.random.default_rng(seed)
+    img = rng.uniform(0, 1, size).astype(np.float32)
+    img[:5] = np.round(img[:5] * 255) / 255      # ties in some rows
+    reg = reg_factory(half=half, clahe=False)
+    reg.set_images(img, img[::-1].copy(), preprocess=False)
+    g = O.round_half(img) if half else img
+    r, hist, cdf, bins = O.rank_transform(g)
+    mean, spread = O.image_stats(g)
+    gr, ghist, gcdf, (gm, gs) = reg.rank(0)
+    assert gm.tobytes() == np.float32(mean).tobytes() and gs.tobytes() == np.float32(spread).tobytes()
+    assert np.array_equal(ghist, hist)
+    assert np.array_equal(gcdf, cdf)
+    assert np.array_equal(gr.view(np.uint32), r.view(np.uint32))
+
+
+def test_features_match_oracle(reg_factory):
+    mv, fx, _ = _pair((160, 192), (7, -5), seed=4)
+    for mode in ("laplacian", "three", "cubic"):
+        reg = reg_factory(half=False, smi_features=mode)
+        reg.set_images(mv, fx, preprocess=False)
+        for side, img in ((0, mv), (1, fx)):
+            p = O.prepare(img, mode)
+            f = reg.features(side)
+            assert np.abs(f - p["feat"]).max() < 2e-4, mode
+
+
+@pytest.mark.parametrize("n,planes", [(32, 3), (64, 2), (256, 2), (1024, 1), (2048, 1), (4096, 1)])
+def test_fft2_matches_numpy(reg_factory, n, planes):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((planes, n, n)) + 1j * rng.standard_normal((planes, n, n))).astype(np.complex64)
+    reg = reg_factory()
+    ref = np.fft.fft2(x.astype(np.complex128))
+    y = reg.fft2(x.copy())
+    assert np.abs(y - ref).max() / np.abs(ref).max() < 2e-6
+    back = reg.fft2(y.copy(), inverse=True)
+    assert np.abs(back - x).max() / np.abs(x).max() < 2e-6
+
+
+def _map_case(reg_factory, mv, fx, H, exact=True, calibrated=False, mode="laplacian", cap=1024, min_overlap=0.0):
+    reg = reg_factory(half=False, smi_features=mode, exact=exact, calibrated=calibrated, map_res=cap,
+                      min_overlap=min_overlap)
+    reg.set_images(mv, fx, preprocess=False)
+    if min_overlap > 0:
+        sm = reg.shift_map(H)
+    else:
+        sm = reg.shift_map(H, exact=exact, calibrated=calibrated, resolution=cap)
+    pa, pb = O.prepare(mv, mode), O.prepare(fx, mode)
+    om, info = O.shift_map(pa, pb, H, exact=exact, calibrated=calibrated, cap=cap, min_overlap=min_overlap)
+    return sm, om, info
+
+
+@pytest.mark.parametrize("exact,calibrated,mode", [(True, False, "laplacian"), (True, False, "three"),
+                                                   (False, False, "laplacian"), (True, True, "laplacian"),
+                                                   (True, False, "cubic")])
+def test_shift_map_256_config0(reg_factory, exact, calibrated, mode):
+    mv, fx, Ht = _pair((256, 256), (17, -9))
+    sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), exact, calibrated, mode)
+    assert (sm.n, sm.ny, sm.cw, sm.ch) == (512, 512, 256, 256)
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same, ((sm.dx, sm.dy), (info["dx"], info["dy"]))
+    assert (sm.dx, sm.dy) == (17.0, -9.0)
+    if calibrated:
+        assert abs(sm.corr_area_px - info["area_px"]) < 1e-3 * info["area_px"]
+
+
+def test_shift_map_min_overlap_25pct(reg_factory):
+    mv, fx, _ = _pair((256, 256), (17, -9))
+    sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), min_overlap=0.25, calibrated=False)
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL and same and (sm.dx, sm.dy) == (17.0, -9.0)
+
+
+def test_shift_map_warped_resampled(reg_factory):
+    """A rotated/scaled pose on a rectangular pair, map capped below the canvas (k x k resample)."""
+    mv, fx, _ = _pair((180, 240), (5, 3), seed=7)
+    c = np.array([[1, 0, 120.5], [0, 1, 90.5], [0, 0, 1.0]])
+    t = np.deg2rad(4.0)
+    R = np.array([[1.03 * np.cos(t), -1.03 * np.sin(t), 0], [1.03 * np.sin(t), 1.03 * np.cos(t), 0], [0, 0, 1]])
+    H = c @ R @ np.linalg.inv(c) @ np.array([[1, 0, -6.0], [0, 1, 4.0], [0, 0, 1]])
+    sm, om, info = _map_case(reg_factory, mv, fx, H, cap=256)
+    assert sm.cw < sm.canvas[0]
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same
+
+
+@pytest.mark.parametrize("mode", ["laplacian", "three"])
+def test_shift_map_1024_config1(reg_factory, mode):
+    """1024² pair, 8-bit intensities, shift (113, -57), full-resolution 2048² FFT."""
+    mv, fx, _ = _pair((1024, 1024), (113, -57), seed=11, quantize=True)
+    sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), mode=mode, cap=2048)
+    assert (sm.n, sm.ny, sm.cw, sm.ch) == (2048, 2048, 1024, 1024)
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same and (sm.dx, sm.dy) == (113.0, -57.0)
+
+
+def test_moments_and_score(reg_factory):
+    mv, fx, Ht = _pair((200, 220), (6, -4), seed=5)
+    H = np.array([[1.01, 0.02, -5.0], [-0.015, 0.99, 3.5], [1e-5, -2e-5, 1.0]])
+    for sym in (False, True):
+        reg = reg_factory(half=False, symmetric=sym, exact=True)
+        reg.set_images(mv, fx, preprocess=False)
+        pa, pb = O.prepare(mv), O.prepare(fx)
+        m = reg.moments(H)
+        om = O.pose_moments(pa["feat"], pb["feat"], H)
+        assert np.abs(m - om).max() < 1e-4 * np.abs(om).max()
+        s = reg.score(H)
+        mean, fwd, inv = O.pose_score(pa["feat"], pb["feat"], H, exact=True, symmetric=sym)
+        assert abs(s.fwd - fwd) < 1e-4 * abs(fwd)
+        assert abs(s.mean - mean) < 1e-4 * abs(mean)
+        assert (s.inv is None) == (not sym)
+
+
+def test_errors_fail_loudly(reg_factory):
+    reg = reg_factory()
+    with pytest.raises(RuntimeError, match="NoImages"):
+        reg.shift_map(np.eye(3), exact=True, calibrated=False)
+    with pytest.raises(RuntimeError, match="BadImage"):
+        reg.set_moving(np.zeros((4, 4), np.float32), preprocess=False)
+    with pytest.raises(RuntimeError, match="Unsupported"):
+        reg.set_moving(np.zeros((16, 16), np.float32), preprocess=True)
+    with pytest.raises(RuntimeError, match="UnknownSetting"):
+        reg.configure(nope=1)

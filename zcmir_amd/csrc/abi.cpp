@@ -1,0 +1,356 @@
+// The C ABI (include/zcsmi.h). Names, argument meaning, result structs and error behaviour
+// follow the reference's exports (src/exports.zig, src/native_api.zig): 0 (or a size) on
+// success, -1 on failure with the reason for zc_error. Exceptions never cross this boundary.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "engine.hpp"
+
+using zc::Engine;
+using zc::Mat3;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+size_t copy_out(const std::string& s, char* buf, size_t cap) {
+  const size_t n = std::min(s.size(), cap);
+  if (buf && n) std::memcpy(buf, s.data(), n);
+  return n;
+}
+
+Mat3 mat(const double* H) {
+  Mat3 m;
+  for (int i = 0; i < 9; ++i) m[i] = H[i];
+  return m;
+}
+
+template <typename F>
+int32_t guard(Engine* e, const char* what, F&& f) {
+  if (!e) return -1;
+  try {
+    (void)hipSetDevice(e->dev);
+    f();
+    return 0;
+  } catch (const std::exception& ex) {
+    e->err = std::string(what) + ": " + ex.what();
+    return -1;
+  }
+}
+
+int32_t unsupported(Engine* e, const char* what) {
+  if (e) e->err = std::string(what) + ": Unsupported (not part of the SMI hot path this library implements)";
+  return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* zc_version(void) { return "0.1.4+mi355x"; }
+
+void* zc_create(const char* dev) {
+  try {
+    int d = 0;
+    if (dev && *dev) {
+      char* end = nullptr;
+      const long v = std::strtol(dev, &end, 10);
+      if (end && *end == '\0') d = (int)v;   // a device index; any other text (a path) is ignored
+    }
+    return new Engine(d);
+  } catch (const std::exception& ex) {
+    g_create_error = ex.what();
+    return nullptr;
+  }
+}
+
+size_t zc_create_error(char* buf, size_t cap) { return copy_out(g_create_error, buf, cap); }
+
+void zc_destroy(void* p) { delete static_cast<Engine*>(p); }
+
+size_t zc_adapter(void* p, char* buf, size_t cap) {
+  Engine* e = static_cast<Engine*>(p);
+  return e ? copy_out(e->device_name, buf, cap) : 0;
+}
+
+size_t zc_error(void* p, char* buf, size_t cap) {
+  Engine* e = static_cast<Engine*>(p);
+  if (!e) return 0;
+  const size_t n = copy_out(e->err, buf, cap);
+  e->err.clear();
+  return n;
+}
+
+int32_t zc_configure(void* p, const char* json, size_t len) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "configure", [&] { e->set.apply(json, len); });
+}
+
+size_t zc_settings(void* p, char* buf, size_t cap) {
+  Engine* e = static_cast<Engine*>(p);
+  if (!e) return 0;
+  const std::string j = e->set.to_json();
+  copy_out(j, buf, cap);
+  return j.size();
+}
+
+int32_t zc_set_image(void* p, uint32_t side, const float* data, uint32_t w, uint32_t h) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "set_image", [&] { e->set_image((int)side, data, (int)w, (int)h, false); });
+}
+
+int32_t zc_set_image_gray(void* p, uint32_t side, const float* data, uint32_t w, uint32_t h, uint32_t preprocess) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "set_image_gray", [&] { e->set_image((int)side, data, (int)w, (int)h, preprocess != 0); });
+}
+
+int32_t zc_set_image_rgba(void* p, uint32_t side, const uint8_t* rgba, uint32_t w, uint32_t h) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "set_image_rgba", [&] {
+    // RGBA images are always preprocessed (exports.zig:60-63): allowed only with preprocessing off
+    std::vector<float> g((size_t)w * h);
+    for (size_t i = 0; i < g.size(); ++i)
+      g[i] = (0.299f * rgba[4 * i] + 0.587f * rgba[4 * i + 1] + 0.114f * rgba[4 * i + 2]) / 255.0f;
+    e->set_image((int)side, g.data(), (int)w, (int)h, true);
+  });
+}
+
+void zc_image_size(void* p, uint32_t side, uint32_t out[2]) {
+  Engine* e = static_cast<Engine*>(p);
+  if (e && side < 2 && e->sides[side].has) {
+    out[0] = (uint32_t)e->sides[side].w;
+    out[1] = (uint32_t)e->sides[side].h;
+  } else {
+    out[0] = out[1] = 0;
+  }
+}
+
+int32_t zc_work_image(void* p, uint32_t side, float* out, size_t len) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "work_image", [&] {
+    if (side > 1 || !e->sides[side].has) throw std::runtime_error("NoImage");
+    const auto& s = e->sides[side];
+    const size_t n = (size_t)s.w * s.h;
+    if (len < n) throw std::runtime_error("BufferTooSmall");
+    ZC_HIP(hipMemcpyAsync(out, s.gray.p, n * 4, hipMemcpyDeviceToHost, e->st));
+    e->sync();
+  });
+}
+
+int32_t zc_features(void* p, uint32_t side, float* out, size_t len) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "features", [&] {
+    if (side > 1 || !e->sides[side].has) throw std::runtime_error("NoImage");
+    const auto& s = e->sides[side];
+    const size_t n = 4 * (size_t)s.w * s.h;
+    if (len < n) throw std::runtime_error("BufferTooSmall");
+    ZC_HIP(hipMemcpyAsync(out, s.feat.p, n * 4, hipMemcpyDeviceToHost, e->st));
+    e->sync();
+  });
+}
+
+int32_t zc_moments(void* p, const double* H, double* out) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "moments", [&] { e->moments(0, 1, mat(H), out); });
+}
+
+int32_t zc_score(void* p, const double* H, double* out) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "score", [&] {
+    const bool exact = e->set.b("exact");
+    double m[45];
+    const Mat3 Hm = mat(H);
+    e->moments(0, 1, Hm, m);
+    const double fwd = Engine::score_from_moments(m, exact, 1e-4);
+    if (!e->set.b("symmetric")) {
+      out[0] = fwd;
+      out[1] = fwd;
+      out[2] = std::numeric_limits<double>::quiet_NaN();
+      return;
+    }
+    e->moments(1, 0, zc::inv3(Hm), m);
+    const double inv = Engine::score_from_moments(m, exact, 1e-4);
+    out[0] = 0.5 * (fwd + inv);
+    out[1] = fwd;
+    out[2] = inv;
+  });
+}
+
+int32_t zc_shift_map(void* p, const double* H, uint32_t flags, uint32_t cap, zc::ShiftResult* out, float* map,
+                     size_t map_len) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "shift_map", [&] {
+    const auto m = e->shift_map(mat(H), (flags & 1) != 0, (flags & 2) != 0, cap == 0 ? 1024 : cap, 0.0);
+    if (map) e->read_map(map, map_len, m);
+    e->fill_shift_result(m, out);
+  });
+}
+
+int32_t zc_shift_map_set(void* p, const double* H, zc::ShiftResult* out, float* map, size_t map_len) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "shift_map", [&] {
+    const std::string metric = e->set.e("metric");
+    if (metric != "smi") throw std::runtime_error("Unsupported metric " + metric + " (smi only)");
+    const auto m = e->shift_map(mat(H), e->set.b("exact"), e->set.b("calibrated"), e->set.u("map_res"),
+                                e->set.f("min_overlap"));
+    if (map) e->read_map(map, map_len, m);
+    e->fill_shift_result(m, out);
+  });
+}
+
+void zc_set_pose(void* p, const double* H) {
+  Engine* e = static_cast<Engine*>(p);
+  if (e) e->H = mat(H);
+}
+
+void zc_get_pose(void* p, double* H) {
+  Engine* e = static_cast<Engine*>(p);
+  if (e)
+    for (int i = 0; i < 9; ++i) H[i] = e->H[i];
+}
+
+size_t zc_events(void* p, char* buf, size_t cap) {
+  (void)p;
+  (void)buf;
+  (void)cap;
+  return 0;
+}
+
+int32_t zc_profile(void* p, uint32_t on) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "profile", [&] {
+    e->sync();
+    e->prof.on = on != 0;
+    e->prof.reset();
+  });
+}
+
+size_t zc_profile_report(void* p, char* buf, size_t cap) {
+  Engine* e = static_cast<Engine*>(p);
+  if (!e) return 0;
+  const std::string r = e->prof.report();
+  copy_out(r, buf, cap);
+  return r.size();
+}
+
+void zc_gpu_counts(void* p, uint32_t out[4]) {
+  Engine* e = static_cast<Engine*>(p);
+  if (!e) return;
+  for (int i = 0; i < 4; ++i) {
+    out[i] = e->counts[i];
+    e->counts[i] = 0;
+  }
+}
+
+void zc_cancel(void* p) {
+  Engine* e = static_cast<Engine*>(p);
+  if (e) e->cancel = true;
+}
+
+void zc_swap(void* p) {
+  Engine* e = static_cast<Engine*>(p);
+  if (!e) return;
+  std::swap(e->sides[0], e->sides[1]);
+}
+
+int32_t zc_fft2(void* p, float* data, uint32_t n, uint32_t planes, uint32_t inverse, uint32_t four_step) {
+  Engine* e = static_cast<Engine*>(p);
+  (void)four_step;
+  return guard(e, "fft2", [&] { e->fft2(data, (int)n, (int)planes, inverse != 0); });
+}
+
+void zc_result_sizes(uint32_t out[4]) {
+  out[0] = sizeof(zc::ShiftResult);
+  out[1] = sizeof(zc::RsResult);
+  out[2] = sizeof(zc::GradResult);
+  out[3] = 0;
+}
+
+// ── inspection and benchmark hooks of this library (no reference counterpart) ──
+int32_t zc_rank(void* p, uint32_t side, float* rank, size_t len, uint32_t* hist, uint32_t* cdf, float* stats) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "rank", [&] {
+    if (side > 1 || !e->sides[side].has) throw std::runtime_error("NoImage");
+    const auto& s = e->sides[side];
+    const size_t n = (size_t)s.w * s.h;
+    if (rank) {
+      if (len < n) throw std::runtime_error("BufferTooSmall");
+      ZC_HIP(hipMemcpyAsync(rank, s.rank.p, n * 4, hipMemcpyDeviceToHost, e->st));
+      e->sync();
+    }
+    if (hist) std::memcpy(hist, s.hist, sizeof s.hist);
+    if (cdf) std::memcpy(cdf, s.cdf, sizeof s.cdf);
+    if (stats) {
+      stats[0] = s.mean;
+      stats[1] = s.spread;
+    }
+  });
+}
+
+// `iters` shift maps of H back to back, no readback between them; the mean time per map (ms,
+// HIP events on the engine's stream) into ms_per_map.
+int32_t zc_bench_shift_map(void* p, const double* H, uint32_t flags, uint32_t cap, uint32_t iters,
+                           double* ms_per_map) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "bench_shift_map", [&] {
+    const auto m0 = e->shift_map(mat(H), (flags & 1) != 0, (flags & 2) != 0, cap, 0.0);
+    (void)m0;
+    hipEvent_t a, b;
+    ZC_HIP(hipEventCreate(&a));
+    ZC_HIP(hipEventCreate(&b));
+    ZC_HIP(hipEventRecord(a, e->st));
+    for (uint32_t i = 0; i < iters; ++i) e->shift_map(mat(H), (flags & 1) != 0, (flags & 2) != 0, cap, 0.0);
+    ZC_HIP(hipEventRecord(b, e->st));
+    ZC_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    ZC_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms_per_map = iters ? ms / iters : 0.0;
+  });
+}
+
+// ── reference entry points outside the SMI hot path: present so a ctypes binding of the
+//    reference's full symbol list loads, each failing with "Unsupported" ──
+#define ZC_UNSUPPORTED(name, ...)                                  \
+  int32_t name(__VA_ARGS__) { return unsupported((Engine*)e, #name); }
+
+ZC_UNSUPPORTED(zc_compile_wgsl, void* e, const char*, size_t, const char*, size_t, char*, size_t)
+ZC_UNSUPPORTED(zc_pg_info, void* e, double*)
+ZC_UNSUPPORTED(zc_pg_maps, void* e, uint32_t, float*, size_t)
+ZC_UNSUPPORTED(zc_pg_descriptors, void* e, uint32_t, float*, size_t)
+ZC_UNSUPPORTED(zc_bench_moments, void* e, uint32_t, uint32_t)
+ZC_UNSUPPORTED(zc_match_scores, void* e, float*, size_t)
+ZC_UNSUPPORTED(zc_detect, void* e, void*)
+ZC_UNSUPPORTED(zc_match, void* e, void*)
+ZC_UNSUPPORTED(zc_auto, void* e, void*)
+ZC_UNSUPPORTED(zc_keypoints, void* e, uint32_t, float*, size_t)
+ZC_UNSUPPORTED(zc_keypoint_frames, void* e, uint32_t, float*, size_t)
+ZC_UNSUPPORTED(zc_matches, void* e, uint32_t*, size_t)
+ZC_UNSUPPORTED(zc_set_ffd, void* e, const float*, size_t)
+ZC_UNSUPPORTED(zc_ffd_gradient, void* e, const double*, double*, double*, size_t)
+ZC_UNSUPPORTED(zc_canvas, void* e, const double*, uint32_t, void*)
+ZC_UNSUPPORTED(zc_overlay, void* e, const double*, uint32_t, void*, float*, float*, size_t)
+ZC_UNSUPPORTED(zc_tile_heat, void* e, const double*, uint32_t, uint32_t, double*, float*, float*, size_t)
+ZC_UNSUPPORTED(zc_displacement, void* e, float*, size_t)
+ZC_UNSUPPORTED(zc_rs_map, void* e, const double*, const double*, uint32_t, void*, float*, float*, float*, size_t)
+ZC_UNSUPPORTED(zc_search, void* e, void*)
+ZC_UNSUPPORTED(zc_climb, void* e, void*)
+ZC_UNSUPPORTED(zc_gradient, void* e, const double*, uint32_t, void*)
+
+int64_t zc_export_warp(void* e, const char*, size_t, void*, size_t) { return unsupported((Engine*)e, "zc_export_warp"); }
+size_t zc_get_ffd(void* e, float*, size_t) {
+  (void)e;
+  return 0;
+}
+uint32_t zc_fit_check(void* e, const double*) {
+  (void)e;
+  return 7;   // "it fails the sanity check": keypoint fits are not part of this library
+}
+void zc_tune_parts(uint32_t, uint32_t, uint32_t, uint32_t) {}
+void zc_fft_tune(uint32_t, uint32_t) {}
+void zc_fft_tune_wg(uint32_t, uint32_t, uint32_t) {}
+
+}  // extern "C"

@@ -1,0 +1,514 @@
+// Engine implementation and the C ABI (extern "C" zc_* entry points).
+//
+// Every entry point returns 0 (or a size) on success and a negative value on failure, with the
+// reason kept for zc_error (exports.zig:1-16). No C++ exception crosses the ABI.
+#include "engine.hpp"
+#include "plan.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+
+namespace zc {
+
+static constexpr double kRidge = 1e-4;   // smi.zig:18
+static constexpr int kMinN = 12;          // smi.zig:17
+static constexpr int kMaxFft = 4096;      // longest FFT line the kernels implement
+
+// ── profiling ─────────────────────────────────────────────────────────────────────────────
+hipEvent_t Prof::get() {
+  if (pool_.empty()) {
+    hipEvent_t e;
+    ZC_HIP(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = pool_.back();
+  pool_.pop_back();
+  return e;
+}
+void Prof::begin(hipStream_t st) {
+  cur_ = get();
+  ZC_HIP(hipEventRecord(cur_, st));
+}
+void Prof::end(const char* name, hipStream_t st) {
+  hipEvent_t b = get();
+  ZC_HIP(hipEventRecord(b, st));
+  pending_.push_back({name, cur_, b});
+  cur_ = nullptr;
+}
+void Prof::collect() {
+  for (auto& p : pending_) {
+    float ms = 0;
+    ZC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& a = acc_[p.name];
+    a.first += 1;
+    a.second += ms;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+std::string Prof::report() const {
+  std::ostringstream o;
+  for (const auto& kv : acc_) {
+    char line[256];
+    std::snprintf(line, sizeof line, "%s %ld %.6f %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second,
+                  kv.second.second / std::max(1L, kv.second.first));
+    o << line;
+  }
+  return o.str();
+}
+Prof::~Prof() {
+  for (auto e : pool_) (void)hipEventDestroy(e);
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+}
+
+#define LAUNCH(name, call) \
+  do {                     \
+    pre();                 \
+    call;                  \
+    post(name);            \
+  } while (0)
+
+// ── engine ────────────────────────────────────────────────────────────────────────────────
+Engine::Engine(int device) : dev(device) {
+  int n = 0;
+  ZC_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) throw std::runtime_error("no HIP device " + std::to_string(device));
+  ZC_HIP(hipSetDevice(dev));
+  ZC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipDeviceProp_t p;
+  ZC_HIP(hipGetDeviceProperties(&p, dev));
+  device_name = std::string(p.name) + " (" + p.gcnArchName + ")";
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(dev);
+  if (st) (void)hipStreamSynchronize(st);
+  for (auto& s : sides) {
+    s.gray.release();
+    s.rank.release();
+    s.feat.release();
+    s.mag.release();
+    s.stack.release();
+  }
+  for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
+                  &tmp_u32, &tmp_i64})
+    b->release();
+  for (auto& kv : tw) kv.second.release();
+  if (st) (void)hipStreamDestroy(st);
+}
+
+void Engine::sync() {
+  ZC_HIP(hipStreamSynchronize(st));
+  ++counts[3];
+  if (prof.on) prof.collect();
+}
+
+const float2* Engine::twiddles(int L) {
+  auto it = tw.find(L);
+  if (it != tw.end()) return it->second.as<float2>();
+  std::vector<float2> h(L);
+  for (int t = 0; t < L; ++t) {
+    const double a = -2.0 * M_PI * (double)t / (double)L;
+    h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  DBuf& b = tw[L];
+  b.ensure(sizeof(float2) * L);
+  ZC_HIP(hipMemcpy(b.p, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+  return b.as<float2>();
+}
+
+void Engine::need_images() {
+  if (!has_images()) throw std::runtime_error("NoImages");
+}
+
+void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) {
+  if (s < 0 || s > 1 || w < 8 || h < 8 || !data) throw std::runtime_error("BadImage");
+  if (preprocess && (set.b("clahe") || set.b("band") || set.b("invert")))
+    throw std::runtime_error(
+        "Unsupported: preprocessing (CLAHE / band pass / invert) is not part of this library; pass "
+        "preprocess=False or configure clahe=False, band=False, invert=False");
+  Side& sd = sides[s];
+  const size_t n = (size_t)w * h;
+  sd.w = w;
+  sd.h = h;
+  sd.has = false;
+  sd.preprocess = preprocess;
+  sd.gray.ensure(n * 4);
+  sd.rank.ensure(n * 4);
+  sd.feat.ensure(n * 16);
+  sd.mag.ensure(n * 4);
+  sd.stack.ensure(n * 20);
+  ZC_HIP(hipMemcpyAsync(sd.gray.p, data, n * 4, hipMemcpyHostToDevice, st));
+  prepare(s);
+  sd.has = true;
+}
+
+// Smi.prepare (smi.zig:348-400) for SMI features, with exact image statistics (see the oracle's
+// definition: order-independent integer sums, so the rank bins are reproducible bit for bit).
+void Engine::prepare(int s) {
+  Side& sd = sides[s];
+  const int64_t n = (int64_t)sd.w * sd.h;
+  float* gray = sd.gray.as<float>();
+  sd.half = set.b("half");
+  sd.mode = set.e("smi_features");
+  if (sd.half) LAUNCH("round_half", launch_round_half(gray, n, st));
+  // max |x| → the fixed-point scale
+  tmp_u32.ensure(4 * 1024);
+  tmp_i64.ensure(8 * 4);
+  uint32_t* acc32 = tmp_u32.as<uint32_t>();
+  ZC_HIP(hipMemsetAsync(acc32, 0, 4, st));
+  LAUNCH("maxabs", launch_maxabs(gray, n, acc32, st));
+  uint32_t maxbits = 0;
+  ZC_HIP(hipMemcpyAsync(&maxbits, acc32, 4, hipMemcpyDeviceToHost, st));
+  sync();
+  float maxabs;
+  std::memcpy(&maxabs, &maxbits, 4);
+  int e = 0;
+  if (maxabs > 0) std::frexp((double)maxabs, &e);
+  auto fixed_sum = [&](int squared, float mean, int K) -> double {
+    long long* hi = tmp_i64.as<long long>();
+    unsigned long long* lo = reinterpret_cast<unsigned long long*>(hi + 1);
+    ZC_HIP(hipMemsetAsync(hi, 0, 16, st));
+    LAUNCH("fixsum", launch_fixsum(gray, n, std::ldexp(1.0, K), mean, squared, hi, lo, st));
+    long long h2[2];
+    ZC_HIP(hipMemcpyAsync(h2, hi, 16, hipMemcpyDeviceToHost, st));
+    sync();
+    const __int128 S = (__int128)h2[0] * ((__int128)1 << 32) + (__int128)(unsigned long long)h2[1];
+    const long long shi = (long long)(S >> 64);
+    const unsigned long long slo = (unsigned long long)S;
+    const double v = ((double)shi * 18446744073709551616.0 + (double)slo) / (double)n;
+    return std::ldexp(v, -K);
+  };
+  const int K = 62 - e;
+  sd.mean = (float)fixed_sum(0, 0.0f, K);
+  const double var = fixed_sum(1, sd.mean, 60 - 2 * e);
+  sd.spread = (float)std::sqrt(std::max(var, 1e-12));
+  // histogram → CDF (host, exact) → rank
+  uint32_t* hist = acc32 + 16;
+  uint32_t* cdf = acc32 + 16 + 272;
+  ZC_HIP(hipMemsetAsync(hist, 0, 257 * 4, st));
+  LAUNCH("hist", launch_hist(gray, n, sd.mean, sd.spread, hist, st));
+  ZC_HIP(hipMemcpyAsync(sd.hist, hist, 257 * 4, hipMemcpyDeviceToHost, st));
+  sync();
+  uint64_t accu = 0;
+  for (int i = 0; i < 256; ++i) {
+    sd.cdf[i] = (uint32_t)accu;
+    accu += sd.hist[i];
+  }
+  sd.cdf[256] = (uint32_t)accu;
+  sd.cdf[257] = 0;
+  ZC_HIP(hipMemcpyAsync(cdf, sd.cdf, 258 * 4, hipMemcpyHostToDevice, st));
+  LAUNCH("rank", launch_rank(gray, n, sd.mean, sd.spread, hist, cdf, sd.rank.as<float>(), st));
+  // features and their moments
+  const int mode = sd.mode == "laplacian" ? 0 : (sd.mode == "three" ? 1 : 2);
+  const int nb = feats_blocks(sd.w, sd.h);
+  parts.ensure((size_t)nb * 14 * 8);
+  sums.ensure(2048 * 8);
+  LAUNCH("feats", launch_feats(sd.rank.as<float>(), sd.w, sd.h, mode, sd.feat.as<float>(), sd.mag.as<float>(),
+                               parts.as<double>(), st));
+  LAUNCH("sum_parts", launch_sum_parts(parts.as<double>(), nb, 14, sums.as<double>(), st));
+  double m[14];
+  ZC_HIP(hipMemcpyAsync(m, sums.p, 14 * 8, hipMemcpyDeviceToHost, st));
+  sync();
+  // global ridge whitening (smi.wgsl:554-599), FP64
+  const double nn = (double)n;
+  double mu[4], G[4][4];
+  for (int i = 0; i < 4; ++i) mu[i] = m[i] / nn;
+  auto tk = [](int i, int j) {
+    const int a = std::min(i, j), b = std::max(i, j);
+    return a * 4 - (a * (a + 1)) / 2 + b;
+  };
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) G[i][j] = m[4 + tk(i, j)] / nn - mu[i] * mu[j];
+  const double tr = G[0][0] + G[1][1] + G[2][2] + G[3][3];
+  const double ridge = kRidge * std::max(tr * 0.25, 1e-12);
+  for (int i = 0; i < 4; ++i) G[i][i] += ridge;
+  double L[4][4] = {};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double sacc = G[i][j];
+      for (int k = 0; k < j; ++k) sacc -= L[i][k] * L[j][k];
+      if (i == j) L[i][i] = std::sqrt(std::max(sacc, 1e-12));
+      else L[i][j] = sacc / L[j][j];
+    }
+  WhitenArgs wa{};
+  for (int c = 0; c < 4; ++c)
+    for (int i = 0; i < 4; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) v -= L[i][k] * wa.X[k * 4 + c];
+      wa.X[i * 4 + c] = v / L[i][i];
+    }
+  for (int i = 0; i < 4; ++i) wa.mu[i] = mu[i];
+  LAUNCH("whiten", launch_whiten(sd.feat.as<float>(), sd.stack.as<float>(), n, wa, st));
+  ensure_acf(s);
+}
+
+// Normalized feature autocorrelation (smi.wgsl:936-975, 1574-1580).
+void Engine::ensure_acf(int s) {
+  Side& sd = sides[s];
+  const int nb = acf_blocks(sd.w, sd.h);
+  parts.ensure((size_t)nb * 1156 * 8);
+  sums.ensure(2048 * 8);
+  LAUNCH("acf", launch_acf(sd.feat.as<float>(), sd.w, sd.h, parts.as<double>(), st));
+  LAUNCH("sum_parts", launch_sum_parts(parts.as<double>(), nb, 1156, sums.as<double>(), st));
+  std::vector<double> raw(1156);
+  ZC_HIP(hipMemcpyAsync(raw.data(), sums.p, 1156 * 8, hipMemcpyDeviceToHost, st));
+  sync();
+  for (int lag = 0; lag < 289; ++lag) {
+    const int dx = lag % 17 - 8, dy = lag / 17 - 8;
+    const double cnt = std::max(1.0, (double)(sd.w - std::abs(dx)) * (double)(sd.h - std::abs(dy)));
+    for (int p = 0; p < 4; ++p) sd.acf[lag * 4 + p] = raw[lag * 4 + p] / cnt;
+  }
+  double zero[4];
+  for (int p = 0; p < 4; ++p) zero[p] = sd.acf[144 * 4 + p];
+  for (int lag = 0; lag < 289; ++lag)
+    for (int p = 0; p < 4; ++p) sd.acf[lag * 4 + p] /= std::max(zero[p], 1e-12);
+}
+
+// Bartlett correlation area of the pair, px² (smi.wgsl:1582-1595).
+double Engine::pair_area() const {
+  double s = 0;
+  for (int k = 0; k < 289; ++k) {
+    double ra = 0, rb = 0;
+    for (int i = 0; i < 4; ++i) {
+      ra += sides[0].acf[k * 4 + i];
+      rb += sides[1].acf[k * 4 + i];
+    }
+    s += ra * rb;
+  }
+  return std::max(1.0, s / 16.0);
+}
+
+// The dense shift map of pose H (smi.zig:540-572; engine.zig:795-819).
+MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, double min_overlap) {
+  need_images();
+  Side& A = sides[0];
+  Side& B = sides[1];
+  if (A.mode != B.mode) throw std::runtime_error("FeatureModeMismatch: set both images with the same smi_features");
+  const bool three = A.mode == "three";
+  const int plan = (exact ? 1 : 0) + (three ? 2 : 0);
+  const CorrPlan& pi = kPlans[plan];
+  MapOut m{};
+  m.exact = exact;
+  m.calib = exact && calib;
+  m.fr = reg_frame(Hp, A.w, A.h, B.w, B.h);
+  const Mat3 Hc = canvas_h(Hp, m.fr.ox, m.fr.oy);
+  const bool ladder = set.e("fft_sizes") == "ladder";
+  if (cap == 0) {
+    m.g = auto_shift_plan(m.fr.w, m.fr.h);
+  } else {
+    const uint32_t capn = largest_smooth(std::min<uint32_t>(cap, kMaxFft), ladder);
+    m.g = linear_corr_plan_rect(m.fr.w, m.fr.h, capn, ladder);
+  }
+  // the kernels take power-of-two lengths: a longer FFT only pads (a linear correlation's
+  // values per lag do not depend on the length once it is >= 2 cells - 1)
+  m.g.nx = pow2_ceil(m.g.nx);
+  m.g.ny = pow2_ceil(m.g.ny);
+  if (m.g.nx > (uint32_t)kMaxFft || m.g.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
+  const int W = m.fr.w, Hh = m.fr.h;
+  const int nx = (int)m.g.nx, ny = (int)m.g.ny, cw = (int)m.g.cw, ch = (int)m.g.ch;
+  const int64_t N = (int64_t)nx * ny;
+  const size_t cn = (size_t)W * Hh;
+  canvas_a.ensure(cn * 20);
+  canvas_b.ensure(cn * 20);
+  spec_a.ensure((size_t)N * 8 * pi.n_cplx);
+  spec_b.ensure((size_t)N * 8 * pi.n_cplx);
+  work.ensure((size_t)N * 8 * pi.n_out);
+  corr.ensure((size_t)N * 4 * pi.n_corr);
+  map.ensure((size_t)N * 4);
+  const int nparts = combine_blocks(N);
+  arg_part.ensure((size_t)nparts * 8);
+  res.ensure(16);
+  // canvas stacks
+  double Hi[9];
+  const double det = det3(Hc);
+  if (std::fabs(det) >= 1e-12) {
+    const Mat3 inv = {(Hc[4] * Hc[8] - Hc[5] * Hc[7]) / det, (Hc[2] * Hc[7] - Hc[1] * Hc[8]) / det,
+                      (Hc[1] * Hc[5] - Hc[2] * Hc[4]) / det, (Hc[5] * Hc[6] - Hc[3] * Hc[8]) / det,
+                      (Hc[0] * Hc[8] - Hc[2] * Hc[6]) / det, (Hc[2] * Hc[3] - Hc[0] * Hc[5]) / det,
+                      (Hc[3] * Hc[7] - Hc[4] * Hc[6]) / det, (Hc[1] * Hc[6] - Hc[0] * Hc[7]) / det,
+                      (Hc[0] * Hc[4] - Hc[1] * Hc[3]) / det};
+    for (int i = 0; i < 9; ++i) Hi[i] = inv[i];
+  } else {
+    for (int i = 0; i < 9; ++i) Hi[i] = 0;   // every q.z = 0: the warped stack is empty
+  }
+  const float2* y_tw = twiddles(ny);
+  LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
+  LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
+  bool ok = true;
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st));
+  LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
+  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st));
+  LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
+  if (!ok) throw std::runtime_error("UnsupportedFftLength");
+  const uint32_t floor_n = min_overlap > 0 ? (uint32_t)std::floor(min_overlap * (double)cw * (double)ch) : 0u;
+  CombineArgs ca{};
+  ca.exact = exact;
+  ca.calib = m.calib;
+  ca.nf = three ? 3 : 4;
+  ca.min_n = (float)std::max<uint32_t>(kMinN, floor_n);
+  ca.ridge = kRidge;
+  const double gcell = (double)W / (double)cw;
+  ca.area = std::max(1.0, pair_area() / (gcell * gcell));
+  LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st));
+  LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
+  ZC_HIP(hipMemcpyAsync(m.res, res.p, 12, hipMemcpyDeviceToHost, st));
+  ++counts[2];
+  ++counts[1];
+  sync();
+  return m;
+}
+
+void Engine::read_map(float* out, size_t len, const MapOut& m) {
+  const size_t nn = (size_t)m.g.nx * m.g.ny;
+  if (len < nn) throw std::runtime_error("MapTooSmall");
+  ZC_HIP(hipMemcpyAsync(out, map.p, nn * 4, hipMemcpyDeviceToHost, st));
+  ++counts[2];
+  sync();
+}
+
+void Engine::fill_shift_result(const MapOut& m, ShiftResult* r) const {
+  std::memset(r, 0, sizeof *r);
+  r->n = m.g.nx;
+  r->n_y = m.g.ny;
+  r->cw = m.g.cw;
+  r->ch = m.g.ch;
+  r->canvas_w = (uint32_t)m.fr.w;
+  r->canvas_h = (uint32_t)m.fr.h;
+  r->ox = m.fr.ox;
+  r->oy = m.fr.oy;
+  r->peak = m.res[0];
+  r->zero = m.res[2];
+  r->corr_area_px = pair_area();
+  r->exact = m.exact;
+  r->calibrated = m.calib;
+  uint32_t idx;
+  std::memcpy(&idx, &m.res[1], 4);
+  const uint64_t nn = (uint64_t)m.g.nx * m.g.ny;
+  if (idx < nn) {
+    // peakLag (engine.zig:789-793)
+    const int64_t py = idx / m.g.nx, px = idx % m.g.nx;
+    const int64_t lx = px <= m.g.nx / 2 ? px : px - (int64_t)m.g.nx;
+    const int64_t ly = py <= m.g.ny / 2 ? py : py - (int64_t)m.g.ny;
+    r->peak_index = idx;
+    r->dx = (double)lx * ((double)m.fr.w / (double)m.g.cw);
+    r->dy = (double)ly * ((double)m.fr.h / (double)m.g.ch);
+  }
+}
+
+// The 45 overlap sums of side a's features onto side b's at H (a → b), smi.wgsl:980-1057.
+void Engine::moments(int as, int bs, const Mat3& Hp, double out[45]) {
+  need_images();
+  const Side& A = sides[as];
+  const Side& B = sides[bs];
+  const Mat3 Hi = inv3(Hp);
+  PoseArgs pa;
+  for (int i = 0; i < 9; ++i) pa.Hi[i] = Hi[i];
+  const int nb = moments_blocks(B.w, B.h);
+  parts.ensure((size_t)nb * 45 * 8);
+  sums.ensure(2048 * 8);
+  LAUNCH("pose_moments", launch_pose_moments(A.feat.as<float>(), A.w, A.h, B.feat.as<float>(), B.w, B.h, pa,
+                                             parts.as<double>(), st));
+  LAUNCH("sum_parts", launch_sum_parts(parts.as<double>(), nb, 45, sums.as<double>(), st));
+  ZC_HIP(hipMemcpyAsync(out, sums.p, 45 * 8, hipMemcpyDeviceToHost, st));
+  ++counts[2];
+  sync();
+}
+
+// moments.zig:91-134 scoreFromMoments (the score part), FP64.
+double Engine::score_from_moments(const double* s, bool exact, double ridge) {
+  const double n = s[0];
+  if (!(n >= kMinN)) return 0.0;
+  double ma[4], mb[4], C[4][4];
+  for (int i = 0; i < 4; ++i) {
+    ma[i] = s[1 + i] / n;
+    mb[i] = s[5 + i] / n;
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) C[i][j] = s[9 + i * 4 + j] / n - ma[i] * mb[j];
+  if (!exact) {
+    double phi = 0;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) phi += C[i][j] * C[i][j];
+    const double sc = n * phi;
+    return std::isfinite(sc) ? sc : 0.0;
+  }
+  auto tk = [](int i, int j) {
+    const int a = std::min(i, j), b = std::max(i, j);
+    return a * 4 - (a * (a + 1)) / 2 + b;
+  };
+  double GA[4][4], GB[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      GA[i][j] = s[25 + tk(i, j)] / n - ma[i] * ma[j];
+      GB[i][j] = s[35 + tk(i, j)] / n - mb[i] * mb[j];
+    }
+  const double ra = ridge * std::max((GA[0][0] + GA[1][1] + GA[2][2] + GA[3][3]) / 4, 1e-12);
+  const double rb = ridge * std::max((GB[0][0] + GB[1][1] + GB[2][2] + GB[3][3]) / 4, 1e-12);
+  for (int i = 0; i < 4; ++i) {
+    GA[i][i] += ra;
+    GB[i][i] += rb;
+  }
+  auto chol = [](double (&G)[4][4], double (&L)[4][4]) {
+    std::memset(L, 0, sizeof L);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j <= i; ++j) {
+        double v = G[i][j];
+        for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+        if (i == j) {
+          if (!(v > 0)) return false;
+          L[i][i] = std::sqrt(v);
+        } else {
+          L[i][j] = v / L[j][j];
+        }
+      }
+    return true;
+  };
+  double LA[4][4], LB[4][4];
+  if (!chol(GA, LA) || !chol(GB, LB)) return 0.0;
+  // Φ = ‖L_A⁻¹ C L_B⁻ᵀ‖²_F = tr(G_A⁻¹ C G_B⁻¹ Cᵀ)
+  double X[4][4];
+  for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i) {
+      double v = C[i][j];
+      for (int k = 0; k < i; ++k) v -= LA[i][k] * X[k][j];
+      X[i][j] = v / LA[i][i];
+    }
+  double phi = 0;
+  for (int r = 0; r < 4; ++r) {
+    double y[4];
+    for (int i = 0; i < 4; ++i) {
+      double v = X[r][i];
+      for (int k = 0; k < i; ++k) v -= LB[i][k] * y[k];
+      y[i] = v / LB[i][i];
+      phi += y[i] * y[i];
+    }
+  }
+  const double sc = n * phi;
+  return std::isfinite(sc) ? sc : 0.0;
+}
+
+void Engine::fft2(float* data, int n, int planes, bool inverse) {
+  if (n > kMaxFft || (n & (n - 1)) || n < 32) throw std::runtime_error("FftLength: a power of two, 32..4096");
+  const size_t bytes = (size_t)planes * n * n * 8;
+  work.ensure(bytes);
+  ZC_HIP(hipMemcpyAsync(work.p, data, bytes, hipMemcpyHostToDevice, st));
+  bool ok = true;
+  LAUNCH("fft2", ok = launch_fft2(work.as<float2>(), planes, n, inverse, twiddles(n), st));
+  if (!ok) throw std::runtime_error("UnsupportedFftLength");
+  ZC_HIP(hipMemcpyAsync(data, work.p, bytes, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+}  // namespace zc

@@ -1,0 +1,182 @@
+// The SMI engine behind the C ABI: one HIP device and stream, an image pair whose features stay
+// resident in HBM, and the map pipeline. Mirrors the reference engine's responsibilities for
+// the hot path (engine.zig:136-873): images → features, pose, maps, scores.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+#include "kernels.hpp"
+#include "settings.hpp"
+
+namespace zc {
+
+#define ZC_HIP(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + #expr); \
+  } while (0)
+
+// A device buffer that grows on demand (contents are not preserved).
+struct DBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) ZC_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    ZC_HIP(hipMalloc(&p, bytes));
+    cap = bytes;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// ABI result structs: layouts identical to the reference's (engine.zig:32-87, search.zig:30-41,
+// climb.zig:7-12; python/zcmir/_native.py:14-58).
+struct ShiftResult {
+  uint32_t n, cw, ch, canvas_w, canvas_h;
+  int32_t ox, oy;
+  uint32_t peak_index;
+  double dx, dy, peak, zero, corr_area_px;
+  uint32_t exact, calibrated, n_y, _pad;
+};
+static_assert(sizeof(ShiftResult) == 88, "ShiftResult layout");
+
+struct RsResult {
+  uint32_t n, n_th, n_lam, peak_index;
+  double dlam, r0, r1, cx, cy, peak, zero, dth, dsg;
+  uint32_t symmetric, calibrated;
+};
+static_assert(sizeof(RsResult) == 96, "RsResult layout");
+
+struct GradResult {
+  double score, fwd, inv, n;
+  double grad[8];
+  double hess[64];
+  uint32_t has_hess, nk;
+};
+static_assert(sizeof(GradResult) == 616, "GradResult layout");
+
+struct SearchResult {
+  double H[9];
+  double score, prev;
+  uint32_t improved, count_a, count_b, _pad;
+};
+
+struct ClimbResult {
+  double H[9];
+  double score;
+  uint32_t iterations, moved;
+};
+
+// Per-kernel timing with HIP events on the engine's stream (zc_profile / zc_profile_report).
+class Prof {
+ public:
+  bool on = false;
+  void begin(hipStream_t st);
+  void end(const char* name, hipStream_t st);
+  void collect();   // after a synchronization
+  std::string report() const;
+  void reset() { acc_.clear(); }
+  ~Prof();
+
+ private:
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> pool_;
+  std::vector<Pending> pending_;
+  hipEvent_t cur_ = nullptr;
+  std::map<std::string, std::pair<long, double>> acc_;
+  hipEvent_t get();
+};
+
+struct Side {
+  int w = 0, h = 0;
+  bool has = false;
+  bool preprocess = false;
+  DBuf gray, rank, feat, mag, stack;
+  float mean = 0, spread = 1;
+  uint32_t hist[257] = {};
+  uint32_t cdf[258] = {};
+  double acf[289 * 4] = {};
+  std::string mode;
+  bool half = false;
+};
+
+struct MapOut {
+  Frame fr;
+  GridPlan g;
+  bool exact, calib;
+  float res[3];
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+
+  int dev;
+  hipStream_t st = nullptr;
+  Settings set;
+  Side sides[2];
+  Mat3 H = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::string err;
+  std::atomic<bool> cancel{false};
+  Prof prof;
+  uint32_t counts[4] = {0, 0, 0, 0};   // dispatches, submits, reads, waits
+  std::string device_name;
+
+  void set_image(int side, const float* data, int w, int h, bool preprocess);
+  bool has_images() const { return sides[0].has && sides[1].has; }
+  void need_images();
+  double pair_area() const;
+
+  // map of H into the engine's map buffer; returns its plan and [peak, idx, map[0]]
+  MapOut shift_map(const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap);
+  void fill_shift_result(const MapOut& m, ShiftResult* out) const;
+  void read_map(float* out, size_t len, const MapOut& m);
+
+  void moments(int a_side, int b_side, const Mat3& H, double out[45]);
+  // SMI from 45 sums (moments.zig:91-134), FP64
+  static double score_from_moments(const double* st, bool exact, double ridge);
+
+  void fft2(float* data, int n, int planes, bool inverse);
+
+  void sync();
+  // launch bookkeeping
+  void pre() { if (prof.on) prof.begin(st); }
+  void post(const char* name) {
+    ++counts[0];
+    if (prof.on) prof.end(name, st);
+    ZC_HIP(hipGetLastError());
+  }
+
+  DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
+  std::map<int, DBuf> tw;
+  const float2* twiddles(int L);
+  void ensure_acf(int s);
+
+ private:
+  void prepare(int s);
+};
+
+}  // namespace zc

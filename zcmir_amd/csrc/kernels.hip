@@ -1,0 +1,949 @@
+// SMI hot-path kernels for gfx950 (CDNA4, wave64).
+//
+// Prepare (once per image): exact integer image statistics → bit-exact rank transform →
+// Sobel/Laplacian features with per-block moments → FP64 ridge whitening → premultiplied stack;
+// feature autocorrelation for calibrated maps.
+// Map (per pose): canvas warp/paste → fused resample + real-plane pack + row FFT → column FFT →
+// Hermitian-unpacking cross-spectrum (all correlations of a bin pair in one pass) → column IFFT →
+// row IFFT + extract → FP64 per-lag whitened score + block argmax → argmax finish.
+// Pose moments: the 45 overlap sums at a pose (FP32 two-sum compensated per thread, FP64 across).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <cstdint>
+#include <cmath>
+#include <algorithm>
+
+#include "fft.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace zc {
+
+static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// ── block reductions (wave64 shuffles, then LDS across waves, fixed order) ──────────────────
+template <typename V, int M, int T>
+__device__ __forceinline__ void block_sum(V (&v)[M], V* out, int tid) {
+  constexpr int W = T / 64;
+  __shared__ V sm[W][M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    V x = v[m];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    v[m] = x;
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) sm[wv][m] = v[m];
+  }
+  __syncthreads();
+  for (int m = tid; m < M; m += T) {
+    V s = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s += sm[w][m];
+    out[m] = s;
+  }
+}
+
+// ── prepare ───────────────────────────────────────────────────────────────────────────────
+__global__ void k_round_half(float* x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = __half2float(__float2half_rn(x[i]));
+}
+
+__global__ __launch_bounds__(256) void k_maxabs(const float* __restrict__ x, int64_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    m = max(m, __float_as_uint(fabsf(x[i])));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_down((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// Exact fixed-point sums: term = trunc(v · 2^K), v = x or f32(x - mean)^2 (exact in f64),
+// split into a signed high and an unsigned low 32-bit part (integer atomics: order-free).
+__global__ __launch_bounds__(256) void k_fixsum(const float* __restrict__ x, int64_t n, double scale, float mean,
+                                                int squared, long long* hi_acc, unsigned long long* lo_acc) {
+  long long hi = 0;
+  unsigned long long lo = 0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double v;
+    if (squared) {
+      const double d = (double)__fsub_rn(x[i], mean);
+      v = d * d;
+    } else {
+      v = (double)x[i];
+    }
+    const long long t = (long long)(v * scale);
+    hi += t >> 32;
+    lo += (unsigned long long)(t & 0xffffffffLL);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    hi += __shfl_down(hi, off, 64);
+    lo += __shfl_down(lo, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd((unsigned long long*)hi_acc, (unsigned long long)hi);
+    atomicAdd(lo_acc, lo);
+  }
+}
+
+__device__ __forceinline__ uint32_t rank_bin(float x, float mean, float spread) {
+  const float d = __fsub_rn(x, mean);
+  const float z = __fdiv_rn(d, fmaxf(spread, 1e-8f));
+  const float t = __fmul_rn(__fadd_rn(z, 4.0f), 32.0f);
+  return (uint32_t)fminf(fmaxf(t, 0.0f), 255.0f);
+}
+
+__global__ __launch_bounds__(256) void k_hist(const float* __restrict__ x, int64_t n, float mean, float spread,
+                                              uint32_t* hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    atomicAdd(&h[rank_bin(x[i], mean, spread)], 1u);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_rank(const float* __restrict__ x, int64_t n, float mean, float spread,
+                                              const uint32_t* __restrict__ hist, const uint32_t* __restrict__ cdf,
+                                              float* rank) {
+  const float tot = (float)max(cdf[256], 1u);
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const uint32_t b = rank_bin(x[i], mean, spread);
+    const float c = (float)cdf[b], hb = (float)hist[b];
+    const float r = __fdiv_rn(__fadd_rn(c, __fmul_rn(0.5f, hb)), tot);
+    rank[i] = __fsub_rn(__fmul_rn(r, 2.0f), 1.0f);
+  }
+}
+
+__device__ __forceinline__ constexpr int tri_k(int i, int j) {
+  return (i < j ? i : j) * 4 - ((i < j ? i : j) * ((i < j ? i : j) + 1)) / 2 + (i < j ? j : i);
+}
+
+// [r, r², |∇r|, f3] (Sobel, clamped borders) and |∇r|; 16×16 tiles; per-block moments:
+// Σ f_i (4) and Σ f_i f_j (i ≤ j, 10) in FP64.
+__global__ __launch_bounds__(256) void k_feats(const float* __restrict__ rank, int w, int h, int mode,
+                                               float* feat, float* mag, double* part) {
+  __shared__ float tile[18][19];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+  for (int k = tid; k < 324; k += 256) {
+    const int lx = k % 18, ly = k / 18;
+    const int gx = min(max(x0 + lx - 1, 0), w - 1), gy = min(max(y0 + ly - 1, 0), h - 1);
+    tile[ly][lx] = rank[(int64_t)gy * w + gx];
+  }
+  __syncthreads();
+  double acc[14];
+#pragma unroll
+  for (int m = 0; m < 14; ++m) acc[m] = 0.0;
+  const int gx = x0 + tx, gy = y0 + ty;
+  if (gx < w && gy < h) {
+    auto at = [&](int dx, int dy) { return tile[ty + 1 + dy][tx + 1 + dx]; };
+    const float r = at(0, 0);
+    const float sx = -at(-1, -1) + at(1, -1) - 2.0f * at(-1, 0) + 2.0f * at(1, 0) - at(-1, 1) + at(1, 1);
+    const float sy = -at(-1, -1) - 2.0f * at(0, -1) - at(1, -1) + at(-1, 1) + 2.0f * at(0, 1) + at(1, 1);
+    const float g = sqrtf(sx * sx + sy * sy);
+    float f3 = 0.0f;
+    if (mode == 0) f3 = at(-1, 0) + at(1, 0) + at(0, -1) + at(0, 1) - 4.0f * r;
+    else if (mode == 2) f3 = r * r * r;
+    const int64_t n = (int64_t)w * h, i = (int64_t)gy * w + gx;
+    const float f[4] = {r, r * r, g, f3};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) feat[p * n + i] = f[p];
+    mag[i] = g;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      acc[a] = f[a];
+#pragma unroll
+      for (int b = a; b < 4; ++b) acc[4 + tri_k(a, b)] = (double)f[a] * (double)f[b];
+    }
+  }
+  const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+  block_sum<double, 14, 256>(acc, part + (int64_t)blk * 14, tid);
+}
+
+// Column sums of an nparts × m matrix, fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int nparts, int m,
+                                                   double* out) {
+  __shared__ double sm[256];
+  for (int c = 0; c < m; ++c) {
+    double s = 0.0;
+    for (int r = threadIdx.x; r < nparts; r += 256) s += part[(int64_t)r * m + c];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if ((int)threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sm[0];
+    __syncthreads();
+  }
+}
+
+// f ← L⁻¹ (f − μ) in FP64, and the stack [1, f0..f3].
+__global__ __launch_bounds__(256) void k_whiten(float* feat, float* stack, int64_t n, WhitenArgs a) {
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double x[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) x[p] = (double)feat[p * n + i] - a.mu[p];
+    stack[i] = 1.0f;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q <= p; ++q) s += a.X[p * 4 + q] * x[q];
+      feat[p * n + i] = (float)s;
+      stack[(p + 1) * n + i] = (float)s;
+    }
+  }
+}
+
+// Autocorrelation partial sums Σ_x f_i(x) f_i(x + τ), |τ| ≤ 8 (289 lags × 4 planes), over
+// 32×32 tiles with an 8-px halo (zero outside the image).
+constexpr int ACF_T = 32;
+__global__ __launch_bounds__(256) void k_acf(const float* __restrict__ feat, int w, int h, double* part) {
+  constexpr int S = ACF_T + 16;
+  __shared__ float t[4][S][S + 1];
+  const int x0 = blockIdx.x * ACF_T, y0 = blockIdx.y * ACF_T;
+  const int64_t n = (int64_t)w * h;
+  for (int k = threadIdx.x; k < 4 * S * S; k += 256) {
+    const int p = k / (S * S), r = k % (S * S), ly = r / S, lx = r % S;
+    const int gx = x0 + lx - 8, gy = y0 + ly - 8;
+    t[p][ly][lx] = (gx >= 0 && gy >= 0 && gx < w && gy < h) ? feat[p * n + (int64_t)gy * w + gx] : 0.0f;
+  }
+  __syncthreads();
+  const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+  const int tw_ = min(ACF_T, w - x0), th_ = min(ACF_T, h - y0);
+  for (int c = threadIdx.x; c < 289 * 4; c += 256) {
+    const int lag = c >> 2, p = c & 3;
+    const int dx = lag % 17 - 8, dy = lag / 17 - 8;
+    float acc = 0.0f;
+    for (int yy = 0; yy < th_; ++yy)
+      for (int xx = 0; xx < tw_; ++xx) acc += t[p][yy + 8][xx + 8] * t[p][yy + 8 + dy][xx + 8 + dx];
+    part[(int64_t)blk * 1156 + c] = (double)acc;
+  }
+}
+
+// ── maps: canvas ──────────────────────────────────────────────────────────────────────────
+struct H9 {
+  double m[9];
+};
+
+__global__ __launch_bounds__(256) void k_warp_stack(const float* __restrict__ stack, int sw, int sh, H9 Hi,
+                                                    float* canvas, int cw, int ch) {
+  const int gx = blockIdx.x * 16 + (threadIdx.x & 15), gy = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (gx >= cw || gy >= ch) return;
+  const int64_t n = (int64_t)cw * ch, pix = (int64_t)gy * cw + gx, sn = (int64_t)sw * sh;
+  const double X = gx + 1.0, Y = gy + 1.0;
+  const double qx = Hi.m[0] * X + Hi.m[1] * Y + Hi.m[2];
+  const double qy = Hi.m[3] * X + Hi.m[4] * Y + Hi.m[5];
+  const double qz = Hi.m[6] * X + Hi.m[7] * Y + Hi.m[8];
+  bool ok = fabs(qz) >= 1e-8;
+  double fx = 0.0, fy = 0.0;
+  if (ok) {
+    fx = qx / qz - 1.0;
+    fy = qy / qz - 1.0;
+    ok = fx >= 0.0 && fy >= 0.0 && fx <= (double)(sw - 1) && fy <= (double)(sh - 1);   // false for NaN
+  }
+  int xa = 0, xb = 0, ya = 0, yb = 0;
+  float tx = 0.f, ty = 0.f;
+  if (ok) {
+    const double x0 = floor(fx), y0 = floor(fy);
+    tx = (float)(fx - x0);
+    ty = (float)(fy - y0);
+    xa = min(max((int)x0, 0), sw - 1);
+    xb = min(max((int)x0 + 1, 0), sw - 1);
+    ya = min(max((int)y0, 0), sh - 1);
+    yb = min(max((int)y0 + 1, 0), sh - 1);
+  }
+#pragma unroll
+  for (int p = 0; p < 5; ++p) {
+    float v = 0.0f;
+    if (ok) {
+      const float* s = stack + p * sn;
+      const float a = s[(int64_t)ya * sw + xa], b = s[(int64_t)ya * sw + xb];
+      const float c = s[(int64_t)yb * sw + xa], d = s[(int64_t)yb * sw + xb];
+      v = (a * (1.0f - tx) + b * tx) * (1.0f - ty) + (c * (1.0f - tx) + d * tx) * ty;
+    }
+    canvas[p * n + pix] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_paste_stack(const float* __restrict__ stack, int sw, int sh, int ox, int oy,
+                                                     float* canvas, int cw, int ch) {
+  const int gx = blockIdx.x * 16 + (threadIdx.x & 15), gy = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (gx >= cw || gy >= ch) return;
+  const int64_t n = (int64_t)cw * ch, pix = (int64_t)gy * cw + gx, sn = (int64_t)sw * sh;
+  const int sx = gx + ox, sy = gy + oy;
+  const bool inside = sx >= 0 && sy >= 0 && sx < sw && sy < sh;
+#pragma unroll
+  for (int p = 0; p < 5; ++p) canvas[p * n + pix] = inside ? stack[p * sn + (int64_t)sy * sw + sx] : 0.0f;
+}
+
+// ── maps: pack + row FFT ──────────────────────────────────────────────────────────────────
+struct Resample {
+  int W, H, cw, ch, kx, ky, identity;
+  float fx, fy;
+};
+
+__device__ __forceinline__ float sample_plane(const float* __restrict__ s, int W, int H, float px, float py) {
+  const float x = px - 0.5f, y = py - 0.5f;
+  const float x0f = floorf(x), y0f = floorf(y);
+  const float ax = x - x0f, ay = y - y0f;
+  const int x0 = (int)x0f, y0 = (int)y0f;
+  const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
+  const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
+  const float a = s[(int64_t)ya * W + xa], b = s[(int64_t)ya * W + xb];
+  const float c = s[(int64_t)yb * W + xa], d = s[(int64_t)yb * W + xb];
+  return (a * (1.0f - ax) + b * ax) * (1.0f - ay) + (c * (1.0f - ax) + d * ax) * ay;
+}
+
+// The five stack values of grid cell (x, y): the canvas pixel, or the mean of k×k bilinear
+// taps over the cell's footprint (smi.wgsl:678-705 pack_resample).
+__device__ __forceinline__ void cell_values(const float* __restrict__ canvas, const Resample& r, int x, int y,
+                                            float* s) {
+  const int64_t n = (int64_t)r.W * r.H;
+  if (r.identity) {
+#pragma unroll
+    for (int p = 0; p < 5; ++p) s[p] = canvas[p * n + (int64_t)y * r.W + x];
+    return;
+  }
+#pragma unroll
+  for (int p = 0; p < 5; ++p) s[p] = 0.0f;
+  for (int j = 0; j < r.ky; ++j) {
+    const float py = ((float)y + ((float)j + 0.5f) / (float)r.ky) * r.fy;
+    for (int i = 0; i < r.kx; ++i) {
+      const float px = ((float)x + ((float)i + 0.5f) / (float)r.kx) * r.fx;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) s[p] += sample_plane(canvas + p * n, r.W, r.H, px, py);
+    }
+  }
+  const float inv = 1.0f / (float)(r.kx * r.ky);
+#pragma unroll
+  for (int p = 0; p < 5; ++p) s[p] *= inv;
+}
+
+// Real plane `pl` (0 w, 1-4 w f_i, 5-14 w f_i f_j) from stack samples (smi.wgsl:643-649).
+__device__ __forceinline__ float real_plane(int pl, const float* s) {
+  if (pl < 5) return s[pl];
+  if (s[0] <= 1e-6f) return 0.0f;
+  constexpr int TI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+  constexpr int TJ[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+  const int k = pl - 5;
+  return s[1 + TI[k]] * s[1 + TJ[k]] / s[0];
+}
+
+template <int L>
+struct RowT {
+  static constexpr int T = L >= 4096 ? 512 : (L >= 2048 ? 256 : (L >= 1024 ? 128 : 64));
+  static constexpr int XP = (L + T - 1) / T;
+};
+
+template <int L, int PID>
+__global__ __launch_bounds__(RowT<L>::T) void k_pack_rowfft(const float* __restrict__ canvas, Resample r, int ny,
+                                                            float2* spec, const float2* __restrict__ tw) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int T = RowT<L>::T, XP = RowT<L>::XP;
+  __shared__ float2 lds[padded_len(L)];
+  const int tid = threadIdx.x, y = blockIdx.x;
+  const int64_t N = (int64_t)L * ny;
+  float s[XP][5];
+#pragma unroll
+  for (int u = 0; u < XP; ++u) {
+    const int x = tid + u * T;
+    if (x < r.cw && x < L) cell_values(canvas, r, x, y, s[u]);
+    else {
+#pragma unroll
+      for (int p = 0; p < 5; ++p) s[u][p] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < P.n_cplx; ++c) {
+#pragma unroll
+    for (int u = 0; u < XP; ++u) {
+      const int x = tid + u * T;
+      if (x < L) {
+        const float re = real_plane(P.slot_plane[2 * c], s[u]);
+        const float im = (2 * c + 1 < P.n_real) ? real_plane(P.slot_plane[2 * c + 1], s[u]) : 0.0f;
+        lds[pad(x)] = make_float2(re, im);
+      }
+    }
+    __syncthreads();
+    fft_lds<L, -1, T, 1>(lds, 0, tw, tid);
+    float2* dst = spec + c * N + (int64_t)y * L;
+#pragma unroll
+    for (int u = 0; u < XP; ++u) {
+      const int x = tid + u * T;
+      if (x < L) dst[x] = lds[pad(x)];
+    }
+    __syncthreads();
+  }
+}
+
+// Generic in-place row FFT (tests, benchmarks).
+template <int L, int S>
+__global__ __launch_bounds__(RowT<L>::T) void k_rowfft(float2* data, float scale, const float2* __restrict__ tw) {
+  constexpr int T = RowT<L>::T, XP = RowT<L>::XP;
+  __shared__ float2 lds[padded_len(L)];
+  const int tid = threadIdx.x;
+  float2* row = data + (int64_t)blockIdx.x * L;
+#pragma unroll
+  for (int u = 0; u < XP; ++u) {
+    const int x = tid + u * T;
+    if (x < L) lds[pad(x)] = row[x];
+  }
+  __syncthreads();
+  fft_lds<L, S, T, 1>(lds, 0, tw, tid);
+#pragma unroll
+  for (int u = 0; u < XP; ++u) {
+    const int x = tid + u * T;
+    if (x < L) {
+      const float2 v = lds[pad(x)];
+      row[x] = make_float2(v.x * scale, v.y * scale);
+    }
+  }
+}
+
+// ── column FFT ────────────────────────────────────────────────────────────────────────────
+template <int L>
+struct ColT {
+  static constexpr int C = L >= 4096 ? 4 : (L >= 2048 ? 8 : 16);
+  static constexpr int T = L >= 1024 ? 512 : 256;
+  static constexpr int STR = padded_len(L) + 1;
+};
+
+template <int L, int S>
+__global__ __launch_bounds__(ColT<L>::T) void k_colfft(float2* data, int nx, int rows_in, float scale,
+                                                       const float2* __restrict__ tw) {
+  constexpr int C = ColT<L>::C, T = ColT<L>::T, STR = ColT<L>::STR;
+  __shared__ float2 lds[C * STR];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * C;
+  float2* base = data + (int64_t)blockIdx.y * nx * L + x0;
+  for (int e = tid; e < C * L; e += T) {
+    const int y = e / C, c = e - y * C;
+    lds[c * STR + pad(y)] = y < rows_in ? base[(int64_t)y * nx + c] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  fft_lds<L, S, T, C>(lds, STR, tw, tid);
+  for (int e = tid; e < C * L; e += T) {
+    const int y = e / C, c = e - y * C;
+    const float2 v = lds[c * STR + pad(y)];
+    base[(int64_t)y * nx + c] = make_float2(v.x * scale, v.y * scale);
+  }
+}
+
+// ── cross spectrum ────────────────────────────────────────────────────────────────────────
+// Spectrum of real slot `odd ? 2p+1 : 2p` at bin k from Z = FFT(x_2p + i x_2p+1) at k and −k,
+// times 2 (the ½ is folded into the output scale): even: Z + conj Z₋, odd: (Z − conj Z₋)/i.
+__device__ __forceinline__ float2 unpack2(float2 z, float2 zm, bool odd) {
+  return odd ? make_float2(z.y + zm.y, zm.x - z.x) : make_float2(z.x + zm.x, z.y - zm.y);
+}
+
+template <int PID>
+__global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, const float2* __restrict__ B,
+                                              float2* work, int nx, int ny, float scale) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int NC = P.n_cplx;
+  const int64_t N = (int64_t)nx * ny;
+  const int64_t i = blockIdx.x * 256 + threadIdx.x;
+  const int64_t half_rows = ny / 2 + 1;
+  if (i >= half_rows * nx) return;
+  const int ky = (int)(i / nx), kx = (int)(i - (int64_t)ky * nx);
+  const int my = (ny - ky) % ny, mx = (nx - kx) % nx;
+  if ((ky == 0 || 2 * ky == ny) && kx > nx / 2) return;   // its mirror bin covers it
+  const int64_t k = (int64_t)ky * nx + kx, km = (int64_t)my * nx + mx;
+  const bool self = k == km;
+  float2 za[NC], zam[NC], zb[NC], zbm[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    za[c] = A[c * N + k];
+    zam[c] = A[c * N + km];
+    zb[c] = B[c * N + k];
+    zbm[c] = B[c * N + km];
+  }
+  const float sc = 0.25f * scale;
+#pragma unroll
+  for (int q = 0; q < P.n_out; ++q) {
+    const int c1 = 2 * q, c2 = 2 * q + 1;
+    const int a1 = P.ca[c1], b1 = P.cb[c1];
+    float2 xa = unpack2(za[a1 >> 1], zam[a1 >> 1], a1 & 1), xb = unpack2(zb[b1 >> 1], zbm[b1 >> 1], b1 & 1);
+    float2 ya = unpack2(zam[a1 >> 1], za[a1 >> 1], a1 & 1), yb = unpack2(zbm[b1 >> 1], zb[b1 >> 1], b1 & 1);
+    float2 p = cmulc(xa, xb), pm = cmulc(ya, yb);
+    if (c2 < P.n_corr) {
+      const int a2 = P.ca[c2], b2 = P.cb[c2];
+      const float2 xa2 = unpack2(za[a2 >> 1], zam[a2 >> 1], a2 & 1), xb2 = unpack2(zb[b2 >> 1], zbm[b2 >> 1], b2 & 1);
+      const float2 ya2 = unpack2(zam[a2 >> 1], za[a2 >> 1], a2 & 1), yb2 = unpack2(zbm[b2 >> 1], zb[b2 >> 1], b2 & 1);
+      const float2 p2 = cmulc(xa2, xb2), pm2 = cmulc(ya2, yb2);
+      p = make_float2(p.x - p2.y, p.y + p2.x);       // p + i p2
+      pm = make_float2(pm.x - pm2.y, pm.y + pm2.x);
+    }
+    work[q * N + k] = make_float2(p.x * sc, p.y * sc);
+    if (!self) work[q * N + km] = make_float2(pm.x * sc, pm.y * sc);
+  }
+}
+
+// ── row IFFT + extract ────────────────────────────────────────────────────────────────────
+template <int L, int PID>
+__global__ __launch_bounds__(RowT<L>::T) void k_rowifft_extract(const float2* __restrict__ work, float* corr, int ny,
+                                                                const float2* __restrict__ tw) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int T = RowT<L>::T, XP = RowT<L>::XP;
+  __shared__ float2 lds[padded_len(L)];
+  const int tid = threadIdx.x, y = blockIdx.x, q = blockIdx.y;
+  const int64_t N = (int64_t)L * ny;
+  const float2* row = work + q * N + (int64_t)y * L;
+#pragma unroll
+  for (int u = 0; u < XP; ++u) {
+    const int x = tid + u * T;
+    if (x < L) lds[pad(x)] = row[x];
+  }
+  __syncthreads();
+  fft_lds<L, 1, T, 1>(lds, 0, tw, tid);
+  float* o1 = corr + (int64_t)(2 * q) * N + (int64_t)y * L;
+  float* o2 = corr + (int64_t)(2 * q + 1) * N + (int64_t)y * L;
+  const bool two = 2 * q + 1 < P.n_corr;
+#pragma unroll
+  for (int u = 0; u < XP; ++u) {
+    const int x = tid + u * T;
+    if (x < L) {
+      const float2 v = lds[pad(x)];
+      o1[x] = v.x;
+      if (two) o2[x] = v.y;
+    }
+  }
+}
+
+// ── per-lag score ─────────────────────────────────────────────────────────────────────────
+__device__ __forceinline__ bool better(float v, uint32_t i, float bv, uint32_t bi) {
+  return v > bv || (v == bv && i < bi);
+}
+
+// Wilson–Hilferty normal score of an F(d1, d2) value (smi.wgsl:816-822).
+__device__ __forceinline__ double f_to_z(double f, double d1, double d2) {
+  const double a = 2.0 / (9.0 * d1), b = 2.0 / (9.0 * d2);
+  const double c = cbrt(fmax(f, 0.0));
+  return ((1.0 - b) * c - (1.0 - a)) / sqrt(b * c * c + a);
+}
+
+template <int NF>
+__device__ __forceinline__ bool chol(double (&g)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < NF; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = g[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= g[i][k] * g[j][k];
+      if (i == j) {
+        if (!(s > 0.0)) return false;
+        g[i][i] = sqrt(s);
+      } else {
+        g[i][j] = s / g[j][j];
+      }
+    }
+  }
+  return true;
+}
+
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr, float* map, int64_t nn,
+                                                 CombineArgs a, float* arg_part) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  constexpr int NF = (PID & 2) ? 3 : 4;
+  auto C = [&](int num, int64_t t) -> double { return (double)corr[(int64_t)P.kept_of[num] * nn + t]; };
+  const double n0 = C(0, 0);
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
+    const double n = C(0, t);
+    const double neff = n / fmax(a.area, 1.0);
+    double score = 0.0;
+    if (!(n < floor_n) && !(calib && neff <= 10.0)) {
+      const double den = fmax(n, 1e-8), iden = 1.0 / den;
+      double ma[4], mb[4];
+#pragma unroll
+      for (int i = 0; i < NF; ++i) {
+        ma[i] = C(1 + i, t) * iden;
+        mb[i] = C(5 + i, t) * iden;
+      }
+      if constexpr (!EXACT) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+#pragma unroll
+          for (int j = 0; j < NF; ++j) {
+            const double c = fmin(fmax(C(9 + i * 4 + j, t) * iden - ma[i] * mb[j], -1.0), 1.0);
+            s += c * c;
+          }
+        score = n * s;
+      } else {
+        double ga[4][4], gb[4][4];
+        double tra = 0.0, trb = 0.0;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+#pragma unroll
+          for (int j = 0; j <= i; ++j) {
+            ga[i][j] = C(25 + tri_k(i, j), t) * iden - ma[i] * ma[j];
+            gb[i][j] = C(35 + tri_k(i, j), t) * iden - mb[i] * mb[j];
+          }
+          tra += ga[i][i];
+          trb += gb[i][i];
+        }
+        const double ra = a.ridge * fmax(tra * 0.25, 1e-12), rb = a.ridge * fmax(trb * 0.25, 1e-12);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+          ga[i][i] += ra;
+          gb[i][i] += rb;
+        }
+        if (chol<NF>(ga) && chol<NF>(gb)) {
+          // X = L_A⁻¹ C (columns), then Y = X L_B⁻ᵀ (rows), acc = ‖Y‖²
+          double x[4][4];
+#pragma unroll
+          for (int j = 0; j < NF; ++j)
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+              double s = C(9 + i * 4 + j, t) * iden - ma[i] * mb[j];
+#pragma unroll
+              for (int k = 0; k < i; ++k) s -= ga[i][k] * x[k][j];
+              x[i][j] = s / ga[i][i];
+            }
+          double acc = 0.0;
+#pragma unroll
+          for (int r = 0; r < NF; ++r) {
+            double y[4];
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+              double s = x[r][i];
+#pragma unroll
+              for (int k = 0; k < i; ++k) s -= gb[i][k] * y[k];
+              y[i] = s / gb[i][i];
+              acc += y[i] * y[i];
+            }
+          }
+          score = n * acc;
+          if (calib) {
+            const double q = (double)NF;
+            const double v = fmin(acc, q - 1e-4);
+            const double f = (neff - q - 1.0) / q * v / (q - v);
+            score = fmax(f_to_z(f, q * q, q * (neff - q - 1.0)), 0.0);
+          }
+        }
+      }
+      if (!(score == score) || isinf(score)) score = 0.0;
+    }
+    const float sv = (float)score;
+    map[t] = sv;
+    if (better(sv, (uint32_t)t, bv, bi)) {
+      bv = sv;
+      bi = (uint32_t)t;
+    }
+  }
+  // block argmax
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_argmax_finish(const float* __restrict__ part, int nparts,
+                                                       const float* __restrict__ map, float* res) {
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int g = threadIdx.x; g < nparts; g += 256) {
+    const float v = part[2 * g];
+    const uint32_t i = __float_as_uint(part[2 * g + 1]);
+    if (i != 0xffffffffu && better(v, i, bv, bi)) {
+      bv = v;
+      bi = i;
+    }
+  }
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[0] = sv_[0];
+    res[1] = __uint_as_float(si_[0]);
+    res[2] = map[0];
+  }
+}
+
+// ── pose moments ──────────────────────────────────────────────────────────────────────────
+// FP32 two-sum (Knuth) accumulation: s + c carries the exact running sum to ~2^-48.
+__device__ __forceinline__ void two_sum_add(float& s, float& c, float x) {
+  const float t = s + x;
+  const float z = t - s;
+  c += (s - (t - z)) + (x - z);
+  s = t;
+}
+
+__device__ __forceinline__ float ramp_w(float v, float lo, float hi) {
+  const float a = (v - lo) * 0.5f, b = (hi - v) * 0.5f;
+  return fminf(fmaxf(a, 0.0f), 1.0f) * fminf(fmaxf(b, 0.0f), 1.0f);
+}
+
+__global__ __launch_bounds__(256) void k_pose_moments(const float* __restrict__ fa, int wa, int ha,
+                                                      const float* __restrict__ fb, int wb, int hb, PoseArgs p,
+                                                      double* part) {
+  float s[45], c[45];
+#pragma unroll
+  for (int m = 0; m < 45; ++m) s[m] = c[m] = 0.0f;
+  const int64_t na = (int64_t)wa * ha, nb = (int64_t)wb * hb;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nb; t += (int64_t)gridDim.x * 256) {
+    const int x = (int)(t % wb), y = (int)(t / wb);
+    const double X = x + 1.0, Y = y + 1.0;
+    const double qz = p.Hi[6] * X + p.Hi[7] * Y + p.Hi[8];
+    if (fabs(qz) < 1e-8) continue;
+    const double fxd = (p.Hi[0] * X + p.Hi[1] * Y + p.Hi[2]) / qz - 1.0;
+    const double fyd = (p.Hi[3] * X + p.Hi[4] * Y + p.Hi[5]) / qz - 1.0;
+    if (!(fxd >= 2.0 && fyd >= 2.0 && fxd <= wa - 3.0 && fyd <= ha - 3.0)) continue;   // NaN fails too
+    const float fx = (float)fxd, fy = (float)fyd;
+    const float wt = ramp_w(fx, 2.0f, (float)wa - 3.0f) * ramp_w(fy, 2.0f, (float)ha - 3.0f);
+    if (!(wt > 0.0f)) continue;
+    const int x0 = (int)floorf(fx), y0 = (int)floorf(fy);
+    const float ax = fx - (float)x0, ay = fy - (float)y0;
+    const int xa = min(max(x0, 0), wa - 1), xb = min(max(x0 + 1, 0), wa - 1);
+    const int ya = min(max(y0, 0), ha - 1), yb = min(max(y0 + 1, 0), ha - 1);
+    float va[4], vb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float* f = fa + q * na;
+      const float v00 = f[(int64_t)ya * wa + xa], v10 = f[(int64_t)ya * wa + xb];
+      const float v01 = f[(int64_t)yb * wa + xa], v11 = f[(int64_t)yb * wa + xb];
+      va[q] = (v00 * (1.0f - ax) + v10 * ax) * (1.0f - ay) + (v01 * (1.0f - ax) + v11 * ax) * ay;
+      vb[q] = fb[q * nb + t];
+    }
+    two_sum_add(s[0], c[0], wt);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      two_sum_add(s[1 + i], c[1 + i], wt * va[i]);
+      two_sum_add(s[5 + i], c[5 + i], wt * vb[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) two_sum_add(s[9 + i * 4 + j], c[9 + i * 4 + j], wt * va[i] * vb[j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = i; j < 4; ++j) {
+        two_sum_add(s[25 + tri_k(i, j)], c[25 + tri_k(i, j)], wt * va[i] * va[j]);
+        two_sum_add(s[35 + tri_k(i, j)], c[35 + tri_k(i, j)], wt * vb[i] * vb[j]);
+      }
+  }
+  double acc[45];
+#pragma unroll
+  for (int m = 0; m < 45; ++m) acc[m] = (double)s[m] + (double)c[m];
+  block_sum<double, 45, 256>(acc, part + (int64_t)blockIdx.x * 45, threadIdx.x);
+}
+
+// ═════════════════════════════════════ launchers ═════════════════════════════════════════
+void launch_round_half(float* x, int64_t n, hipStream_t st) {
+  k_round_half<<<min(blocks_for(n, 256), 8192u), 256, 0, st>>>(x, n);
+}
+void launch_maxabs(const float* x, int64_t n, uint32_t* out, hipStream_t st) {
+  k_maxabs<<<min(blocks_for(n, 256), 4096u), 256, 0, st>>>(x, n, out);
+}
+void launch_fixsum(const float* x, int64_t n, double scale, float mean, int squared, long long* hi,
+                   unsigned long long* lo, hipStream_t st) {
+  k_fixsum<<<min(blocks_for(n, 256), 4096u), 256, 0, st>>>(x, n, scale, mean, squared, hi, lo);
+}
+void launch_hist(const float* x, int64_t n, float mean, float spread, uint32_t* hist, hipStream_t st) {
+  k_hist<<<min(blocks_for(n, 256), 2048u), 256, 0, st>>>(x, n, mean, spread, hist);
+}
+void launch_rank(const float* x, int64_t n, float mean, float spread, const uint32_t* hist, const uint32_t* cdf,
+                 float* rank, hipStream_t st) {
+  k_rank<<<min(blocks_for(n, 256), 8192u), 256, 0, st>>>(x, n, mean, spread, hist, cdf, rank);
+}
+int feats_blocks(int w, int h) { return ((w + 15) / 16) * ((h + 15) / 16); }
+int launch_feats(const float* rank, int w, int h, int mode, float* feat, float* mag, double* part, hipStream_t st) {
+  dim3 g((w + 15) / 16, (h + 15) / 16);
+  k_feats<<<g, 256, 0, st>>>(rank, w, h, mode, feat, mag, part);
+  return feats_blocks(w, h);
+}
+void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStream_t st) {
+  k_sum_parts<<<1, 256, 0, st>>>(part, nparts, m, out);
+}
+void launch_whiten(float* feat, float* stack, int64_t n, const WhitenArgs& a, hipStream_t st) {
+  k_whiten<<<min(blocks_for(n, 256), 8192u), 256, 0, st>>>(feat, stack, n, a);
+}
+int acf_blocks(int w, int h) { return ((w + ACF_T - 1) / ACF_T) * ((h + ACF_T - 1) / ACF_T); }
+void launch_acf(const float* feat, int w, int h, double* part, hipStream_t st) {
+  dim3 g((w + ACF_T - 1) / ACF_T, (h + ACF_T - 1) / ACF_T);
+  k_acf<<<g, 256, 0, st>>>(feat, w, h, part);
+}
+
+void launch_warp_stack(const float* stack, int sw, int sh, const double* Hi, float* canvas, int cw, int ch,
+                       hipStream_t st) {
+  H9 h;
+  for (int i = 0; i < 9; ++i) h.m[i] = Hi[i];
+  dim3 g((cw + 15) / 16, (ch + 15) / 16);
+  k_warp_stack<<<g, 256, 0, st>>>(stack, sw, sh, h, canvas, cw, ch);
+}
+void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, float* canvas, int cw, int ch,
+                        hipStream_t st) {
+  dim3 g((cw + 15) / 16, (ch + 15) / 16);
+  k_paste_stack<<<g, 256, 0, st>>>(stack, sw, sh, ox, oy, canvas, cw, ch);
+}
+
+template <int L>
+static void pack_rowfft_L(int plan, const float* canvas, const Resample& r, int ny, float2* spec, const float2* tw,
+                          hipStream_t st) {
+  dim3 g(r.ch);
+  switch (plan) {
+    case 0: k_pack_rowfft<L, 0><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
+    case 1: k_pack_rowfft<L, 1><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
+    case 2: k_pack_rowfft<L, 2><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
+    default: k_pack_rowfft<L, 3><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
+  }
+}
+
+#define ZC_POW2_SWITCH(L, CALL) \
+  switch (L) {                  \
+    case 32: CALL(32); break;   \
+    case 64: CALL(64); break;   \
+    case 128: CALL(128); break; \
+    case 256: CALL(256); break; \
+    case 512: CALL(512); break; \
+    case 1024: CALL(1024); break; \
+    case 2048: CALL(2048); break; \
+    case 4096: CALL(4096); break; \
+    default: return false;      \
+  }
+
+bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int ch, int nx, int ny, float2* spec,
+                        const float2* tw_x, hipStream_t st) {
+  Resample r;
+  r.W = W;
+  r.H = H;
+  r.cw = cw;
+  r.ch = ch;
+  r.fx = (float)W / (float)(cw > 0 ? cw : 1);
+  r.fy = (float)H / (float)(ch > 0 ? ch : 1);
+  r.kx = (int)fminf(fmaxf(ceilf(r.fx - 1e-4f), 1.0f), 8.0f);
+  r.ky = (int)fminf(fmaxf(ceilf(r.fy - 1e-4f), 1.0f), 8.0f);
+  r.identity = (cw == W && ch == H) ? 1 : 0;
+#define CALL(LL) pack_rowfft_L<LL>(plan, canvas, r, ny, spec, tw_x, st)
+  ZC_POW2_SWITCH(nx, CALL)
+#undef CALL
+  return true;
+}
+
+bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
+                   hipStream_t st) {
+#define CALL(LL)                                                                                         \
+  {                                                                                                      \
+    if (nx % ColT<LL>::C) return false;                                                                  \
+    dim3 g(nx / ColT<LL>::C, planes);                                                                    \
+    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, nx, rows_in, 1.0f, tw_y);              \
+    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, nx, rows_in, 1.0f, tw_y);                    \
+  }
+  ZC_POW2_SWITCH(ny, CALL)
+#undef CALL
+  return true;
+}
+
+void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* work, int nx, int ny, hipStream_t st) {
+  const int64_t cnt = (int64_t)(ny / 2 + 1) * nx;
+  const float scale = 1.0f / ((float)nx * (float)ny);
+  const unsigned g = blocks_for(cnt, 256);
+  switch (plan) {
+    case 0: k_cmul<0><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 1: k_cmul<1><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 2: k_cmul<2><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    default: k_cmul<3><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+  }
+}
+
+template <int L>
+static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st) {
+  dim3 g(ny, kPlans[plan].n_out);
+  switch (plan) {
+    case 0: k_rowifft_extract<L, 0><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 1: k_rowifft_extract<L, 1><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 2: k_rowifft_extract<L, 2><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    default: k_rowifft_extract<L, 3><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+  }
+}
+
+bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
+                            hipStream_t st) {
+#define CALL(LL) rowifft_L<LL>(plan, work, corr, ny, tw_x, st)
+  ZC_POW2_SWITCH(nx, CALL)
+#undef CALL
+  return true;
+}
+
+int combine_blocks(int64_t nn) { return (int)std::min<int64_t>((nn + 255) / 256, 4096); }
+
+void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
+                    hipStream_t st) {
+  const int g = combine_blocks(nn);
+  switch (plan) {
+    case 0: k_combine<0><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 1: k_combine<1><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 2: k_combine<2><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    default: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+  }
+}
+
+void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st) {
+  k_argmax_finish<<<1, 256, 0, st>>>(arg_part, nparts, map, res);
+}
+
+bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st) {
+  const float s = inverse ? 1.0f / (float)n : 1.0f;
+#define CALL(LL)                                                                                \
+  {                                                                                             \
+    if (inverse) k_rowfft<LL, 1><<<planes * LL, RowT<LL>::T, 0, st>>>(data, s, tw);             \
+    else k_rowfft<LL, -1><<<planes * LL, RowT<LL>::T, 0, st>>>(data, s, tw);                    \
+    if (n % ColT<LL>::C) return false;                                                          \
+    dim3 g(n / ColT<LL>::C, planes);                                                            \
+    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, n, n, s, tw);                 \
+    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, n, n, s, tw);                        \
+  }
+  ZC_POW2_SWITCH(n, CALL)
+#undef CALL
+  return true;
+}
+
+int moments_blocks(int w, int h) {
+  const int64_t n = (int64_t)w * h;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 256 * 64 - 1) / (256 * 64)));
+}
+
+void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_b, int wb, int hb, const PoseArgs& p,
+                         double* part, hipStream_t st) {
+  k_pose_moments<<<moments_blocks(wb, hb), 256, 0, st>>>(feat_a, wa, ha, feat_b, wb, hb, p, part);
+}
+
+}  // namespace zc

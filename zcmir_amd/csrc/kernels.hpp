@@ -1,0 +1,70 @@
+// Host-side launchers of the SMI kernels (kernels.hip). All launches go on `st`; device
+// pointers only. Sizes are validated by the engine before any launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace zc {
+
+struct WhitenArgs {
+  double X[16];   // lower-triangular L^-1, row-major
+  double mu[4];
+};
+
+// pose sampling: Hinv (fixed pixel → moving pixel, 1-based homogeneous), row-major
+struct PoseArgs {
+  double Hi[9];
+};
+
+// ── prepare ──
+void launch_round_half(float* x, int64_t n, hipStream_t st);
+void launch_maxabs(const float* x, int64_t n, uint32_t* out, hipStream_t st);
+void launch_fixsum(const float* x, int64_t n, double scale, float mean, int squared, long long* hi,
+                   unsigned long long* lo, hipStream_t st);
+void launch_hist(const float* x, int64_t n, float mean, float spread, uint32_t* hist, hipStream_t st);
+void launch_rank(const float* x, int64_t n, float mean, float spread, const uint32_t* hist, const uint32_t* cdf,
+                 float* rank, hipStream_t st);
+// features + per-block partial moments (14 doubles per block); returns the block count
+int launch_feats(const float* rank, int w, int h, int mode, float* feat, float* mag, double* part,
+                 hipStream_t st);
+int feats_blocks(int w, int h);
+void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStream_t st);
+void launch_whiten(float* feat, float* stack, int64_t n, const WhitenArgs& a, hipStream_t st);
+int acf_blocks(int w, int h);
+void launch_acf(const float* feat, int w, int h, double* part, hipStream_t st);
+
+// ── maps ──
+void launch_warp_stack(const float* stack, int sw, int sh, const double* Hi, float* canvas, int cw, int ch,
+                       hipStream_t st);
+void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, float* canvas, int cw, int ch,
+                        hipStream_t st);
+// grid: nx × ny FFT, cw × ch cells resampled from a W × H canvas; writes n_cplx planes of
+// ch rows × nx into spec (plane stride nx·ny). Returns false for an unsupported length.
+bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
+                        float2* spec, const float2* tw_x, hipStream_t st);
+// column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
+bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
+                   hipStream_t st);
+void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* work, int nx, int ny,
+                 hipStream_t st);
+bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
+                            hipStream_t st);
+struct CombineArgs {
+  int exact, calib, nf;
+  float min_n;
+  double area, ridge;
+};
+int combine_blocks(int64_t nn);
+void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
+                    hipStream_t st);
+void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st);
+
+// plain 2-D FFT of `planes` n × n planes (tests, benchmarks)
+bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st);
+
+// ── pose moments (45 sums per pose, per-block partials) ──
+int moments_blocks(int w, int h);
+void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_b, int wb, int hb, const PoseArgs& p,
+                         double* part, hipStream_t st);
+
+}  // namespace zc

@@ -1,0 +1,142 @@
+"""Synthetic image pairs of the SMI workloads (seeded, reproducible).
+
+These are inputs for benchmarks, tests and examples; no part of the compute path depends on
+them. Workloads:
+
+* ``shift_pair``: fixed = Gaussian-filtered noise (sigma 4 px, unit std) + 20 Gaussian blobs,
+  min-max scaled to [0, 1]; moving(x) = 1 - fixed(x - d)^2 + N(0, 0.02), clipped to [0, 1].
+  Built on a larger canvas and cropped, so nothing wraps. The true pose (moving → fixed) is
+  translate(-dx, -dy) and the shift map's peak lag is (dx, dy). ``quantize`` rounds both to
+  k / 255 (8-bit intensities).
+* ``rotscale_pair``: moving = fixed rotated by theta and scaled by s about the image centre
+  (bilinear), then remapped by x^0.5.
+* ``stack_slices``: uneven-size slices, each with a random homography and intensity remap.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def _texture(h: int, w: int, rng: np.random.Generator, n_blobs: int = 20) -> np.ndarray:
+    from scipy.ndimage import gaussian_filter
+
+    f = gaussian_filter(rng.standard_normal((h, w)), 4.0, mode="wrap")
+    f /= max(f.std(), 1e-12)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    for _ in range(n_blobs):
+        cy, cx = rng.uniform(0, h), rng.uniform(0, w)
+        sg = rng.uniform(2.0, max(3.0, min(h, w) / 12.0))
+        amp = rng.uniform(-3.0, 3.0)
+        f += amp * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg * sg))
+    f -= f.min()
+    f /= max(f.max(), 1e-12)
+    return f
+
+
+def shift_pair(size=(256, 256), shift=(17, -9), seed: int = 0, quantize: bool = False,
+               noise: float = 0.02):
+    """(moving, fixed, H_true) float32 images of ``size`` = (h, w) (see module docstring)."""
+    h, w = size
+    dx, dy = shift
+    m = max(abs(dx), abs(dy)) + 8
+    rng = np.random.default_rng(seed)
+    big = _texture(h + 2 * m, w + 2 * m, rng)
+    fixed = big[m:m + h, m:m + w]
+    src = big[m - dy:m - dy + h, m - dx:m - dx + w]
+    moving = 1.0 - src * src + rng.normal(0.0, noise, (h, w))
+    moving = np.clip(moving, 0.0, 1.0)
+    if quantize:
+        fixed = np.round(fixed * 255.0) / 255.0
+        moving = np.round(moving * 255.0) / 255.0
+    H = np.array([[1.0, 0.0, -dx], [0.0, 1.0, -dy], [0.0, 0.0, 1.0]])
+    return moving.astype(np.float32), fixed.astype(np.float32), H
+
+
+def _bilinear(img: np.ndarray, fx: np.ndarray, fy: np.ndarray) -> np.ndarray:
+    h, w = img.shape
+    x0 = np.floor(fx).astype(np.int64)
+    y0 = np.floor(fy).astype(np.int64)
+    ax, ay = fx - x0, fy - y0
+    inside = (fx >= 0) & (fy >= 0) & (fx <= w - 1) & (fy <= h - 1)
+    xa, xb = np.clip(x0, 0, w - 1), np.clip(x0 + 1, 0, w - 1)
+    ya, yb = np.clip(y0, 0, h - 1), np.clip(y0 + 1, 0, h - 1)
+    v = (img[ya, xa] * (1 - ax) + img[ya, xb] * ax) * (1 - ay) + (img[yb, xa] * (1 - ax) + img[yb, xb] * ax) * ay
+    return np.where(inside, v, 0.0)
+
+
+def warp(img: np.ndarray, H: np.ndarray, out_shape) -> np.ndarray:
+    """img (moving frame) resampled into a frame of out_shape through H (moving → out, 1-based)."""
+    ho, wo = out_shape
+    y, x = np.mgrid[0:ho, 0:wo].astype(np.float64)
+    X = np.stack([x + 1, y + 1, np.ones_like(x)])
+    q = np.tensordot(np.linalg.inv(H), X, 1)
+    return _bilinear(np.asarray(img, np.float64), q[0] / q[2] - 1, q[1] / q[2] - 1)
+
+
+def about(c, A):
+    """translate(c) · A · translate(-c) for a 2x2 A (3x3 result)."""
+    T = np.eye(3)
+    T[:2, :2] = A
+    cx, cy = c
+    return np.array([[1, 0, cx], [0, 1, cy], [0, 0, 1.0]]) @ T @ np.array([[1, 0, -cx], [0, 1, -cy], [0, 0, 1.0]])
+
+
+def rot(theta_deg: float, s: float = 1.0) -> np.ndarray:
+    t = math.radians(theta_deg)
+    return s * np.array([[math.cos(t), -math.sin(t)], [math.sin(t), math.cos(t)]])
+
+
+def rotscale_pair(size=(1024, 1024), theta_deg: float = 37.0, scale: float = 1.12, seed: int = 2,
+                  remap=np.sqrt):
+    """(moving, fixed, H_true): moving = fixed rotated theta and scaled s about the centre
+    (bilinear), then remapped. H_true maps moving → fixed (1-based)."""
+    h, w = size
+    rng = np.random.default_rng(seed)
+    fixed = _texture(h, w, rng)
+    c = ((w + 1) / 2.0, (h + 1) / 2.0)
+    T = about(c, rot(theta_deg, scale))          # fixed → moving
+    moving = remap(np.clip(warp(fixed, T, (h, w)), 0.0, 1.0))
+    return moving.astype(np.float32), fixed.astype(np.float32), np.linalg.inv(T)
+
+
+REMAPS = {
+    "1-x^2": lambda x: 1.0 - x * x,
+    "x^0.5": np.sqrt,
+    "sin(pi x)": lambda x: np.sin(np.pi * x),
+}
+
+
+def random_homography(rng: np.random.Generator, w: int, h: int, rot_deg=10.0, scale=(0.9, 1.1),
+                      persp=1e-4, shift_frac=0.03):
+    """A random homography about the image centre: rotation ±rot_deg, isotropic scale, a small
+    translation and perspective terms of magnitude <= persp."""
+    c = ((w + 1) / 2.0, (h + 1) / 2.0)
+    A = rot(rng.uniform(-rot_deg, rot_deg), rng.uniform(*scale))
+    H = about(c, A)
+    H[0, 2] += rng.uniform(-shift_frac, shift_frac) * w
+    H[1, 2] += rng.uniform(-shift_frac, shift_frac) * h
+    P = np.eye(3)
+    P[2, 0] = rng.uniform(-persp, persp)
+    P[2, 1] = rng.uniform(-persp, persp)
+    return about(c, np.eye(2)) @ (np.array([[1, 0, c[0]], [0, 1, c[1]], [0, 0, 1]]) @ P @ np.array([[1, 0, -c[0]], [0, 1, -c[1]], [0, 0, 1.0]])) @ H
+
+
+def stack_slices(n: int = 64, side=(512, 1536), max_aspect: float = 1.5, seed: int = 3):
+    """[(moving, fixed, H_true, remap name)] uneven-size slices (see module docstring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    names = list(REMAPS)
+    for _ in range(n):
+        a = int(rng.integers(side[0], side[1] + 1))
+        asp = rng.uniform(1.0, max_aspect)
+        b = int(max(side[0] // 2, min(side[1], round(a / asp))))
+        h, w = (a, b) if rng.uniform() < 0.5 else (b, a)
+        fixed = _texture(h, w, rng)
+        Ht = random_homography(rng, w, h)         # moving → fixed
+        mv = warp(fixed, np.linalg.inv(Ht), (h, w))
+        name = names[int(rng.integers(len(names)))]
+        moving = np.clip(REMAPS[name](np.clip(mv, 0.0, 1.0)), 0.0, 1.0)
+        out.append((moving.astype(np.float32), fixed.astype(np.float32), Ht, name))
+    return out
