@@ -301,9 +301,9 @@ int32_t zc_bench_shift_map(void* p, const double* H, uint32_t flags, uint32_t ca
     ZC_HIP(hipEventCreate(&a));
     ZC_HIP(hipEventCreate(&b));
     ZC_HIP(hipEventRecord(a, e->st));
-    for (uint32_t i = 0; i < iters; ++i) e->shift_map(mat(H), (flags & 1) != 0, (flags & 2) != 0, cap, 0.0);
+    for (uint32_t i = 0; i < iters; ++i) e->shift_map(mat(H), (flags & 1) != 0, (flags & 2) != 0, cap, 0.0, false);
     ZC_HIP(hipEventRecord(b, e->st));
-    ZC_HIP(hipEventSynchronize(b));
+    e->sync();
     float ms = 0;
     ZC_HIP(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);

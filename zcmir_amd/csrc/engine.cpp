@@ -86,6 +86,7 @@ Engine::Engine(int device) : dev(device) {
   hipDeviceProp_t p;
   ZC_HIP(hipGetDeviceProperties(&p, dev));
   device_name = std::string(p.name) + " (" + p.gcnArchName + ")";
+  ZC_HIP(hipHostMalloc((void**)&res_host, 64, hipHostMallocDefault));
 }
 
 Engine::~Engine() {
@@ -102,6 +103,7 @@ Engine::~Engine() {
                   &tmp_u32, &tmp_i64})
     b->release();
   for (auto& kv : tw) kv.second.release();
+  if (res_host) (void)hipHostFree(res_host);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -288,7 +290,7 @@ double Engine::pair_area() const {
 }
 
 // The dense shift map of pose H (smi.zig:540-572; engine.zig:795-819).
-MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, double min_overlap) {
+MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, double min_overlap, bool wait) {
   need_images();
   Side& A = sides[0];
   Side& B = sides[1];
@@ -363,10 +365,13 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   ca.area = std::max(1.0, pair_area() / (gcell * gcell));
   LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
-  ZC_HIP(hipMemcpyAsync(m.res, res.p, 12, hipMemcpyDeviceToHost, st));
+  ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
   ++counts[2];
   ++counts[1];
-  sync();
+  if (wait) {
+    sync();
+    std::memcpy(m.res, res_host, 12);
+  }
   return m;
 }
 

@@ -151,7 +151,9 @@ class Engine {
   double pair_area() const;
 
   // map of H into the engine's map buffer; returns its plan and [peak, idx, map[0]]
-  MapOut shift_map(const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap);
+  // wait=false: enqueue only (the result lands in res_host at the next synchronization)
+  MapOut shift_map(const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap, bool wait = true);
+  float* res_host = nullptr;   // pinned
   void fill_shift_result(const MapOut& m, ShiftResult* out) const;
   void read_map(float* out, size_t len, const MapOut& m);
 
