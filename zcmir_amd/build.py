@@ -18,7 +18,7 @@ LIB = HERE / "libzcsmi.so"
 ARCH = os.environ.get("ZCSMI_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "engine.cpp", "abi.cpp"]
-HEADERS = ["fft.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp"]
+HEADERS = ["fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -14,6 +14,7 @@
 #include <algorithm>
 
 #include "fft.hpp"
+#include "fft_regs.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
@@ -338,104 +339,110 @@ __device__ __forceinline__ float real_plane(int pl, const float* s) {
   return s[1 + TI[k]] * s[1 + TJ[k]] / s[0];
 }
 
+// Row kernels: LINES rows of length L per workgroup, M = 16 points per thread (L >= 1024: one
+// row, L / 16 threads).
 template <int L>
 struct RowT {
-  static constexpr int T = L >= 4096 ? 512 : (L >= 2048 ? 256 : (L >= 1024 ? 128 : 64));
-  static constexpr int XP = (L + T - 1) / T;
+  static constexpr int LINES = L >= 1024 ? 1 : 1024 / L;
+  static constexpr int M = 16;
+  static constexpr int T = LINES * L / M;
+  static constexpr int STR = padded_len(L);
 };
 
+// Resample + real-plane pack + forward row FFT: rows y < ch of the grid; writes n_cplx spectra
+// rows (plane stride L · ny). The stack values of a thread's 16 points are read once.
 template <int L, int PID>
 __global__ __launch_bounds__(RowT<L>::T) void k_pack_rowfft(const float* __restrict__ canvas, Resample r, int ny,
                                                             float2* spec, const float2* __restrict__ tw) {
   constexpr CorrPlan P = kPlans[PID];
-  constexpr int T = RowT<L>::T, XP = RowT<L>::XP;
-  __shared__ float2 lds[padded_len(L)];
-  const int tid = threadIdx.x, y = blockIdx.x;
+  constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
+  __shared__ float2 lds[LINES * STR];
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x * LINES;
   const int64_t N = (int64_t)L * ny;
-  float s[XP][5];
+  float s[M][5];
+  {
+    float2 dummy[M];
+    int idx = 0;
+    rload<L, M, T, LINES, false>(dummy, tid, [&](int line, int x) {
+      const int y = y0 + line;
+      if (x < r.cw && y < r.ch) cell_values(canvas, r, x, y, s[idx]);
+      else {
 #pragma unroll
-  for (int u = 0; u < XP; ++u) {
-    const int x = tid + u * T;
-    if (x < r.cw && x < L) cell_values(canvas, r, x, y, s[u]);
-    else {
-#pragma unroll
-      for (int p = 0; p < 5; ++p) s[u][p] = 0.0f;
-    }
+        for (int p = 0; p < 5; ++p) s[idx][p] = 0.0f;
+      }
+      ++idx;
+      return make_float2(0.0f, 0.0f);
+    });
   }
 #pragma unroll
   for (int c = 0; c < P.n_cplx; ++c) {
+    float2 v[M];
 #pragma unroll
-    for (int u = 0; u < XP; ++u) {
-      const int x = tid + u * T;
-      if (x < L) {
-        const float re = real_plane(P.slot_plane[2 * c], s[u]);
-        const float im = (2 * c + 1 < P.n_real) ? real_plane(P.slot_plane[2 * c + 1], s[u]) : 0.0f;
-        lds[pad(x)] = make_float2(re, im);
-      }
+    for (int i = 0; i < M; ++i) {
+      const float re = real_plane(P.slot_plane[2 * c], s[i]);
+      const float im = (2 * c + 1 < P.n_real) ? real_plane(P.slot_plane[2 * c + 1], s[i]) : 0.0f;
+      v[i] = make_float2(re, im);
     }
-    __syncthreads();
-    fft_lds<L, -1, T, 1>(lds, 0, tw, tid);
-    float2* dst = spec + c * N + (int64_t)y * L;
-#pragma unroll
-    for (int u = 0; u < XP; ++u) {
-      const int x = tid + u * T;
-      if (x < L) dst[x] = lds[pad(x)];
-    }
-    __syncthreads();
+    rfft<L, M, T, LINES, false, -1>(v, lds, STR, tw, tid);
+    float2* dst = spec + c * N;
+    rstore<L, M, T, LINES, false>(v, tid, [&](int line, int x, float2 val) {
+      const int y = y0 + line;
+      if (y < r.ch) dst[(int64_t)y * L + x] = val;
+    });
+    __syncthreads();   // the next plane's first exchange reuses the LDS
   }
 }
 
-// Generic in-place row FFT (tests, benchmarks).
+// Generic in-place row FFT of `rows` rows (tests, benchmarks).
 template <int L, int S>
-__global__ __launch_bounds__(RowT<L>::T) void k_rowfft(float2* data, float scale, const float2* __restrict__ tw) {
-  constexpr int T = RowT<L>::T, XP = RowT<L>::XP;
-  __shared__ float2 lds[padded_len(L)];
+__global__ __launch_bounds__(RowT<L>::T) void k_rowfft(float2* data, int rows, float scale,
+                                                       const float2* __restrict__ tw) {
+  constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
+  __shared__ float2 lds[LINES * STR];
   const int tid = threadIdx.x;
-  float2* row = data + (int64_t)blockIdx.x * L;
-#pragma unroll
-  for (int u = 0; u < XP; ++u) {
-    const int x = tid + u * T;
-    if (x < L) lds[pad(x)] = row[x];
-  }
-  __syncthreads();
-  fft_lds<L, S, T, 1>(lds, 0, tw, tid);
-#pragma unroll
-  for (int u = 0; u < XP; ++u) {
-    const int x = tid + u * T;
-    if (x < L) {
-      const float2 v = lds[pad(x)];
-      row[x] = make_float2(v.x * scale, v.y * scale);
-    }
-  }
+  const int y0 = blockIdx.x * LINES;
+  float2 v[M];
+  rload<L, M, T, LINES, false>(v, tid, [&](int line, int x) {
+    const int y = y0 + line;
+    return y < rows ? data[(int64_t)y * L + x] : make_float2(0.0f, 0.0f);
+  });
+  rfft<L, M, T, LINES, false, S>(v, lds, STR, tw, tid);
+  rstore<L, M, T, LINES, false>(v, tid, [&](int line, int x, float2 val) {
+    const int y = y0 + line;
+    if (y < rows) data[(int64_t)y * L + x] = make_float2(val.x * scale, val.y * scale);
+  });
 }
 
 // ── column FFT ────────────────────────────────────────────────────────────────────────────
+// C adjacent columns per workgroup (line-fastest mapping: a wave reads C·8-byte row segments),
+// 16 points per thread; logical blocks remapped so column neighbours share an XCD's L2.
 template <int L>
 struct ColT {
   static constexpr int C = L >= 4096 ? 4 : (L >= 2048 ? 8 : 16);
-  static constexpr int T = L >= 1024 ? 512 : 256;
-  static constexpr int STR = padded_len(L) + 1;
+  static constexpr int M = 16;
+  static constexpr int T = C * L / M;
+  static constexpr int STR = line_stride(L, C);
 };
 
 template <int L, int S>
-__global__ __launch_bounds__(ColT<L>::T) void k_colfft(float2* data, int nx, int rows_in, float scale,
+__global__ __launch_bounds__(ColT<L>::T) void k_colfft(float2* data, int nx, int planes, int rows_in, float scale,
                                                        const float2* __restrict__ tw) {
-  constexpr int C = ColT<L>::C, T = ColT<L>::T, STR = ColT<L>::STR;
+  constexpr int C = ColT<L>::C, T = ColT<L>::T, M = ColT<L>::M, STR = ColT<L>::STR;
   __shared__ float2 lds[C * STR];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * C;
-  float2* base = data + (int64_t)blockIdx.y * nx * L + x0;
-  for (int e = tid; e < C * L; e += T) {
-    const int y = e / C, c = e - y * C;
-    lds[c * STR + pad(y)] = y < rows_in ? base[(int64_t)y * nx + c] : make_float2(0.0f, 0.0f);
-  }
-  __syncthreads();
-  fft_lds<L, S, T, C>(lds, STR, tw, tid);
-  for (int e = tid; e < C * L; e += T) {
-    const int y = e / C, c = e - y * C;
-    const float2 v = lds[c * STR + pad(y)];
-    base[(int64_t)y * nx + c] = make_float2(v.x * scale, v.y * scale);
-  }
+  const int groups = nx / C;
+  const int lb = xcd_remap((int)blockIdx.x, groups * planes);
+  const int plane = lb / groups, x0 = (lb - plane * groups) * C;
+  float2* base = data + (int64_t)plane * nx * L + x0;
+  float2 v[M];
+  rload<L, M, T, C, true>(v, tid, [&](int c, int y) {
+    return y < rows_in ? base[(int64_t)y * nx + c] : make_float2(0.0f, 0.0f);
+  });
+  rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
+  rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+    base[(int64_t)y * nx + c] = make_float2(val.x * scale, val.y * scale);
+  });
 }
 
 // ── cross spectrum ────────────────────────────────────────────────────────────────────────
@@ -489,34 +496,33 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
 }
 
 // ── row IFFT + extract ────────────────────────────────────────────────────────────────────
+// Inverse row FFT of complex output q (rows of L) → correlation planes 2q (re) and 2q+1 (im).
 template <int L, int PID>
 __global__ __launch_bounds__(RowT<L>::T) void k_rowifft_extract(const float2* __restrict__ work, float* corr, int ny,
                                                                 const float2* __restrict__ tw) {
   constexpr CorrPlan P = kPlans[PID];
-  constexpr int T = RowT<L>::T, XP = RowT<L>::XP;
-  __shared__ float2 lds[padded_len(L)];
-  const int tid = threadIdx.x, y = blockIdx.x, q = blockIdx.y;
+  constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
+  __shared__ float2 lds[LINES * STR];
+  const int tid = threadIdx.x, q = blockIdx.y;
+  const int y0 = blockIdx.x * LINES;
   const int64_t N = (int64_t)L * ny;
-  const float2* row = work + q * N + (int64_t)y * L;
-#pragma unroll
-  for (int u = 0; u < XP; ++u) {
-    const int x = tid + u * T;
-    if (x < L) lds[pad(x)] = row[x];
-  }
-  __syncthreads();
-  fft_lds<L, 1, T, 1>(lds, 0, tw, tid);
-  float* o1 = corr + (int64_t)(2 * q) * N + (int64_t)y * L;
-  float* o2 = corr + (int64_t)(2 * q + 1) * N + (int64_t)y * L;
+  const float2* src = work + q * N;
+  float2 v[M];
+  rload<L, M, T, LINES, false>(v, tid, [&](int line, int x) {
+    const int y = y0 + line;
+    return y < ny ? src[(int64_t)y * L + x] : make_float2(0.0f, 0.0f);
+  });
+  rfft<L, M, T, LINES, false, 1>(v, lds, STR, tw, tid);
+  float* o1 = corr + (int64_t)(2 * q) * N;
+  float* o2 = corr + (int64_t)(2 * q + 1) * N;
   const bool two = 2 * q + 1 < P.n_corr;
-#pragma unroll
-  for (int u = 0; u < XP; ++u) {
-    const int x = tid + u * T;
-    if (x < L) {
-      const float2 v = lds[pad(x)];
-      o1[x] = v.x;
-      if (two) o2[x] = v.y;
+  rstore<L, M, T, LINES, false>(v, tid, [&](int line, int x, float2 val) {
+    const int y = y0 + line;
+    if (y < ny) {
+      o1[(int64_t)y * L + x] = val.x;
+      if (two) o2[(int64_t)y * L + x] = val.y;
     }
-  }
+  });
 }
 
 // ── per-lag score ─────────────────────────────────────────────────────────────────────────
@@ -818,7 +824,7 @@ void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, floa
 template <int L>
 static void pack_rowfft_L(int plan, const float* canvas, const Resample& r, int ny, float2* spec, const float2* tw,
                           hipStream_t st) {
-  dim3 g(r.ch);
+  dim3 g((r.ch + RowT<L>::LINES - 1) / RowT<L>::LINES);
   switch (plan) {
     case 0: k_pack_rowfft<L, 0><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
     case 1: k_pack_rowfft<L, 1><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
@@ -863,9 +869,9 @@ bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool i
 #define CALL(LL)                                                                                         \
   {                                                                                                      \
     if (nx % ColT<LL>::C) return false;                                                                  \
-    dim3 g(nx / ColT<LL>::C, planes);                                                                    \
-    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, nx, rows_in, 1.0f, tw_y);              \
-    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, nx, rows_in, 1.0f, tw_y);                    \
+    const unsigned g = (unsigned)(nx / ColT<LL>::C * planes);                                            \
+    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, nx, planes, rows_in, 1.0f, tw_y);      \
+    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, nx, planes, rows_in, 1.0f, tw_y);            \
   }
   ZC_POW2_SWITCH(ny, CALL)
 #undef CALL
@@ -886,7 +892,7 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
 
 template <int L>
 static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st) {
-  dim3 g(ny, kPlans[plan].n_out);
+  dim3 g((ny + RowT<L>::LINES - 1) / RowT<L>::LINES, kPlans[plan].n_out);
   switch (plan) {
     case 0: k_rowifft_extract<L, 0><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
     case 1: k_rowifft_extract<L, 1><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
@@ -924,12 +930,14 @@ bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw
   const float s = inverse ? 1.0f / (float)n : 1.0f;
 #define CALL(LL)                                                                                \
   {                                                                                             \
-    if (inverse) k_rowfft<LL, 1><<<planes * LL, RowT<LL>::T, 0, st>>>(data, s, tw);             \
-    else k_rowfft<LL, -1><<<planes * LL, RowT<LL>::T, 0, st>>>(data, s, tw);                    \
+    const int rows = planes * LL;                                                               \
+    const unsigned gr = (unsigned)((rows + RowT<LL>::LINES - 1) / RowT<LL>::LINES);              \
+    if (inverse) k_rowfft<LL, 1><<<gr, RowT<LL>::T, 0, st>>>(data, rows, s, tw);                \
+    else k_rowfft<LL, -1><<<gr, RowT<LL>::T, 0, st>>>(data, rows, s, tw);                       \
     if (n % ColT<LL>::C) return false;                                                          \
-    dim3 g(n / ColT<LL>::C, planes);                                                            \
-    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, n, n, s, tw);                 \
-    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, n, n, s, tw);                        \
+    const unsigned g = (unsigned)(n / ColT<LL>::C * planes);                                    \
+    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, n, planes, n, s, tw);         \
+    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, n, planes, n, s, tw);                \
   }
   ZC_POW2_SWITCH(n, CALL)
 #undef CALL
