@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a copy kernel of this many MiB and exit")
+    ap.add_argument("--fft-cols", type=int, default=0, help="columns per workgroup of the column FFT (0: default)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -83,6 +84,8 @@ def main():
     from zcmir_amd import synth
 
     three = a.corr == 28
+    if a.fft_cols:
+        zcmir_amd._load().zc_fft_tune(0, a.fft_cols)
     size, shift = (1024, 1024), (113, -57)
     mv, fx, _ = synth.shift_pair(size, shift, seed=11, quantize=True)
     reg = zcmir_amd.Registrar(half=False, symmetric=False, smi_features="three" if three else "laplacian")

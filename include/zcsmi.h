@@ -135,6 +135,9 @@ size_t zc_profile_report(void* e, char* buf, size_t cap);
 void zc_gpu_counts(void* e, uint32_t out[4]);
 /* exports.zig:497 */
 void zc_result_sizes(uint32_t out[4]);
+/* exports.zig:95: lines per workgroup of the batched FFTs for rows / columns (0: default), for
+ * FFTs run from now on; process-wide */
+void zc_fft_tune(uint32_t rows, uint32_t cols);
 
 /* ── this library's own hooks (no counterpart in the reference) ── */
 /* the rank transform of a side: rank (w*h f32, nullable), hist[257], cdf[258], stats[2] =
@@ -171,7 +174,6 @@ int32_t zc_tile_heat(void* e, const double H[9], uint32_t grid, uint32_t min_n, 
 int64_t zc_export_warp(void* e, const char* stem, size_t stem_len, void* out, size_t cap);
 int32_t zc_displacement(void* e, float* out, size_t len);
 void zc_tune_parts(uint32_t px_per_thread, uint32_t fill_wg, uint32_t tg_px, uint32_t tg_cap);
-void zc_fft_tune(uint32_t rows, uint32_t cols);
 void zc_fft_tune_wg(uint32_t rows, uint32_t cols, uint32_t balanced);
 
 #ifdef __cplusplus

@@ -148,3 +148,29 @@ def test_errors_fail_loudly(reg_factory):
         reg.set_moving(np.zeros((16, 16), np.float32), preprocess=True)
     with pytest.raises(RuntimeError, match="UnknownSetting"):
         reg.configure(nope=1)
+
+
+@pytest.mark.parametrize("n", [256, 1024, 2048, 4096])
+def test_fft2_every_column_tiling(reg_factory, n):
+    """Each columns-per-workgroup setting of the column FFT (zc_fft_tune) computes the same FFT."""
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    rng = np.random.default_rng(n + 1)
+    x = (rng.standard_normal((1, n, n)) + 1j * rng.standard_normal((1, n, n))).astype(np.complex64)
+    ref = np.fft.fft2(x.astype(np.complex128))
+    reg = reg_factory()
+    ran = 0
+    try:
+        for cols in (1, 2, 4, 8, 16):
+            lib.zc_fft_tune(0, cols)
+            try:
+                y = reg.fft2(x.copy())
+            except RuntimeError as e:
+                assert "UnsupportedFftLength" in str(e)
+                continue
+            ran += 1
+            assert np.abs(y - ref).max() / np.abs(ref).max() < 2e-6, cols
+    finally:
+        lib.zc_fft_tune(0, 0)
+    assert ran >= 2

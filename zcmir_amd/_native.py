@@ -119,6 +119,7 @@ _SIGS = {
     "zc_displacement": ([P, f32p, SZ], I32),
     # exports.zig names the reference binding leaves out
     "zc_fft2": ([P, f32p, U32, U32, U32, U32], I32),
+    "zc_fft_tune": ([U32, U32], None),
     "zc_result_sizes": ([u32p], None),
     # this library's own hooks (no reference counterpart)
     "zc_rank": ([P, U32, f32p, SZ, u32p, u32p, f32p], I32),

@@ -350,7 +350,12 @@ uint32_t zc_fit_check(void* e, const double*) {
   return 7;   // "it fails the sanity check": keypoint fits are not part of this library
 }
 void zc_tune_parts(uint32_t, uint32_t, uint32_t, uint32_t) {}
-void zc_fft_tune(uint32_t, uint32_t) {}
+// exports.zig:95: lines per workgroup of the batched FFT for rows / columns (0: default), for
+// FFTs run from now on (an unsupported value makes those FFTs fail with UnsupportedFftLength)
+void zc_fft_tune(uint32_t rows, uint32_t cols) {
+  zc::FftTune::rows = (int)rows;
+  zc::FftTune::cols = (int)cols;
+}
 void zc_fft_tune_wg(uint32_t, uint32_t, uint32_t) {}
 
 }  // extern "C"

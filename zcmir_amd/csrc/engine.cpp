@@ -348,10 +348,10 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   bool ok = true;
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
   LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
-  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st));
+  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, FftTune::cols));
   LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   const uint32_t floor_n = min_overlap > 0 ? (uint32_t)std::floor(min_overlap * (double)cw * (double)ch) : 0u;
@@ -510,7 +510,7 @@ void Engine::fft2(float* data, int n, int planes, bool inverse) {
   work.ensure(bytes);
   ZC_HIP(hipMemcpyAsync(work.p, data, bytes, hipMemcpyHostToDevice, st));
   bool ok = true;
-  LAUNCH("fft2", ok = launch_fft2(work.as<float2>(), planes, n, inverse, twiddles(n), st));
+  LAUNCH("fft2", ok = launch_fft2(work.as<float2>(), planes, n, inverse, twiddles(n), st, FftTune::cols));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   ZC_HIP(hipMemcpyAsync(data, work.p, bytes, hipMemcpyDeviceToHost, st));
   sync();

@@ -122,6 +122,11 @@ struct Side {
   bool half = false;
 };
 
+// FFT tuning (zc_fft_tune): lines per workgroup for rows / columns, 0 = the default.
+struct FftTune {
+  static inline std::atomic<int> rows{0}, cols{0};
+};
+
 struct MapOut {
   Frame fr;
   GridPlan g;

@@ -417,18 +417,21 @@ __global__ __launch_bounds__(RowT<L>::T) void k_rowfft(float2* data, int rows, f
 // ── column FFT ────────────────────────────────────────────────────────────────────────────
 // C adjacent columns per workgroup (line-fastest mapping: a wave reads C·8-byte row segments),
 // 16 points per thread; logical blocks remapped so column neighbours share an XCD's L2.
-template <int L>
+template <int L, int C>
 struct ColT {
-  static constexpr int C = L >= 4096 ? 4 : (L >= 2048 ? 8 : 16);
   static constexpr int M = 16;
   static constexpr int T = C * L / M;
   static constexpr int STR = line_stride(L, C);
+  static constexpr bool fits = C * STR * 8 <= 160 * 1024 && T <= 1024 && T >= 64;
 };
 
-template <int L, int S>
-__global__ __launch_bounds__(ColT<L>::T) void k_colfft(float2* data, int nx, int planes, int rows_in, float scale,
-                                                       const float2* __restrict__ tw) {
-  constexpr int C = ColT<L>::C, T = ColT<L>::T, M = ColT<L>::M, STR = ColT<L>::STR;
+// lines (columns) per workgroup of the column FFT: the default, or a tuned value that fits
+inline int col_lines_default(int L) { return L >= 4096 ? 2 : (L >= 2048 ? 4 : (L >= 1024 ? 8 : 16)); }
+
+template <int L, int S, int C>
+__global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx, int planes, int rows_in, float scale,
+                                                          const float2* __restrict__ tw) {
+  constexpr int T = ColT<L, C>::T, M = ColT<L, C>::M, STR = ColT<L, C>::STR;
   __shared__ float2 lds[C * STR];
   const int tid = threadIdx.x;
   const int groups = nx / C;
@@ -443,6 +446,31 @@ __global__ __launch_bounds__(ColT<L>::T) void k_colfft(float2* data, int nx, int
   rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
     base[(int64_t)y * nx + c] = make_float2(val.x * scale, val.y * scale);
   });
+}
+
+template <int L, int S, int C>
+static bool colfft_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st) {
+  if constexpr (!ColT<L, C>::fits) {
+    return false;
+  } else {
+    if (nx % C) return false;
+    const unsigned g = (unsigned)(nx / C * planes);
+    k_colfft<L, S, C><<<g, ColT<L, C>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw);
+    return true;
+  }
+}
+
+template <int L, int S>
+static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, int lines,
+                     hipStream_t st) {
+  switch (lines > 0 ? lines : col_lines_default(L)) {
+    case 1: return colfft_LC<L, S, 1>(data, planes, nx, rows_in, scale, tw, st);
+    case 2: return colfft_LC<L, S, 2>(data, planes, nx, rows_in, scale, tw, st);
+    case 4: return colfft_LC<L, S, 4>(data, planes, nx, rows_in, scale, tw, st);
+    case 8: return colfft_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st);
+    case 16: return colfft_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st);
+    default: return false;
+  }
 }
 
 // ── cross spectrum ────────────────────────────────────────────────────────────────────────
@@ -537,8 +565,9 @@ __device__ __forceinline__ double f_to_z(double f, double d1, double d2) {
   return ((1.0 - b) * c - (1.0 - a)) / sqrt(b * c * c + a);
 }
 
+// In-place Cholesky of the lower triangle of g (NF x NF); rd[i] = 1 / L_ii. False if not PD.
 template <int NF>
-__device__ __forceinline__ bool chol(double (&g)[4][4]) {
+__device__ __forceinline__ bool chol(double (&g)[4][4], double (&rd)[4]) {
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
 #pragma unroll
@@ -548,9 +577,11 @@ __device__ __forceinline__ bool chol(double (&g)[4][4]) {
       for (int k = 0; k < j; ++k) s -= g[i][k] * g[j][k];
       if (i == j) {
         if (!(s > 0.0)) return false;
-        g[i][i] = sqrt(s);
+        const double l = sqrt(s);
+        g[i][i] = l;
+        rd[i] = 1.0 / l;
       } else {
-        g[i][j] = s / g[j][j];
+        g[i][j] = s * rd[j];
       }
     }
   }
@@ -610,7 +641,8 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
           ga[i][i] += ra;
           gb[i][i] += rb;
         }
-        if (chol<NF>(ga) && chol<NF>(gb)) {
+        double ra_[4], rb_[4];
+        if (chol<NF>(ga, ra_) && chol<NF>(gb, rb_)) {
           // X = L_A⁻¹ C (columns), then Y = X L_B⁻ᵀ (rows), acc = ‖Y‖²
           double x[4][4];
 #pragma unroll
@@ -620,7 +652,7 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
               double s = C(9 + i * 4 + j, t) * iden - ma[i] * mb[j];
 #pragma unroll
               for (int k = 0; k < i; ++k) s -= ga[i][k] * x[k][j];
-              x[i][j] = s / ga[i][i];
+              x[i][j] = s * ra_[i];
             }
           double acc = 0.0;
 #pragma unroll
@@ -631,7 +663,7 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
               double s = x[r][i];
 #pragma unroll
               for (int k = 0; k < i; ++k) s -= gb[i][k] * y[k];
-              y[i] = s / gb[i][i];
+              y[i] = s * rb_[i];
               acc += y[i] * y[i];
             }
           }
@@ -865,14 +897,10 @@ bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int
 }
 
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
-                   hipStream_t st) {
-#define CALL(LL)                                                                                         \
-  {                                                                                                      \
-    if (nx % ColT<LL>::C) return false;                                                                  \
-    const unsigned g = (unsigned)(nx / ColT<LL>::C * planes);                                            \
-    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, nx, planes, rows_in, 1.0f, tw_y);      \
-    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, nx, planes, rows_in, 1.0f, tw_y);            \
-  }
+                   hipStream_t st, int lines) {
+#define CALL(LL)                                                                                   \
+  return inverse ? colfft_L<LL, 1>(data, planes, nx, rows_in, 1.0f, tw_y, lines, st)               \
+                 : colfft_L<LL, -1>(data, planes, nx, rows_in, 1.0f, tw_y, lines, st);
   ZC_POW2_SWITCH(ny, CALL)
 #undef CALL
   return true;
@@ -926,18 +954,16 @@ void launch_argmax_finish(const float* arg_part, int nparts, const float* map, f
   k_argmax_finish<<<1, 256, 0, st>>>(arg_part, nparts, map, res);
 }
 
-bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st) {
-  const float s = inverse ? 1.0f / (float)n : 1.0f;
+bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st, int lines) {
+  const float sc = inverse ? 1.0f / (float)n : 1.0f;
 #define CALL(LL)                                                                                \
   {                                                                                             \
     const int rows = planes * LL;                                                               \
     const unsigned gr = (unsigned)((rows + RowT<LL>::LINES - 1) / RowT<LL>::LINES);              \
-    if (inverse) k_rowfft<LL, 1><<<gr, RowT<LL>::T, 0, st>>>(data, rows, s, tw);                \
-    else k_rowfft<LL, -1><<<gr, RowT<LL>::T, 0, st>>>(data, rows, s, tw);                       \
-    if (n % ColT<LL>::C) return false;                                                          \
-    const unsigned g = (unsigned)(n / ColT<LL>::C * planes);                                    \
-    if (inverse) k_colfft<LL, 1><<<g, ColT<LL>::T, 0, st>>>(data, n, planes, n, s, tw);         \
-    else k_colfft<LL, -1><<<g, ColT<LL>::T, 0, st>>>(data, n, planes, n, s, tw);                \
+    if (inverse) k_rowfft<LL, 1><<<gr, RowT<LL>::T, 0, st>>>(data, rows, sc, tw);               \
+    else k_rowfft<LL, -1><<<gr, RowT<LL>::T, 0, st>>>(data, rows, sc, tw);                      \
+    return inverse ? colfft_L<LL, 1>(data, planes, n, n, sc, tw, lines, st)                     \
+                   : colfft_L<LL, -1>(data, planes, n, n, sc, tw, lines, st);                   \
   }
   ZC_POW2_SWITCH(n, CALL)
 #undef CALL

@@ -43,8 +43,9 @@ void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, floa
 bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
                         float2* spec, const float2* tw_x, hipStream_t st);
 // column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
+// lines: columns per workgroup (0: the default for the length)
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
-                   hipStream_t st);
+                   hipStream_t st, int lines = 0);
 void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* work, int nx, int ny,
                  hipStream_t st);
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
@@ -60,7 +61,7 @@ void launch_combine(int plan, const float* corr, float* map, int64_t nn, const C
 void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st);
 
 // plain 2-D FFT of `planes` n × n planes (tests, benchmarks)
-bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st);
+bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st, int lines = 0);
 
 // ── pose moments (45 sums per pose, per-block partials) ──
 int moments_blocks(int w, int h);
