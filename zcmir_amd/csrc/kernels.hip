@@ -422,11 +422,14 @@ struct ColT {
   static constexpr int M = 16;
   static constexpr int T = C * L / M;
   static constexpr int STR = line_stride(L, C);
-  static constexpr bool fits = C * STR * 8 <= 160 * 1024 && T <= 1024 && T >= 64;
+  static constexpr bool fits = C * STR * 8 <= 160 * 1024 && T <= 1024 && T >= 16;
 };
 
 // lines (columns) per workgroup of the column FFT: the default, or a tuned value that fits
 inline int col_lines_default(int L) { return L >= 4096 ? 2 : (L >= 2048 ? 4 : (L >= 1024 ? 8 : 16)); }
+static_assert(ColT<32, 16>::fits && ColT<64, 16>::fits && ColT<128, 16>::fits && ColT<256, 16>::fits &&
+                  ColT<512, 16>::fits && ColT<1024, 8>::fits && ColT<2048, 4>::fits && ColT<4096, 2>::fits,
+              "every default column tiling must be launchable");
 
 template <int L, int S, int C>
 __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx, int planes, int rows_in, float scale,
