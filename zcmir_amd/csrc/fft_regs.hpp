@@ -169,6 +169,28 @@ __device__ __forceinline__ void rfft(float2 (&v)[M], float2* lds, int stride, co
   rstages_from<L, M, T, LINES, LFAST, S, 0>(v, lds, stride, tw, tid);
 }
 
+// G column groups of LINES lines each, time-multiplexing one LDS buffer of LINES lines: every
+// stage computes all groups, then exchanges group 0, group 1, ... through the same LDS (two
+// barriers per group and exchange). Lets a workgroup own G·LINES adjacent columns (wider row
+// segments in memory) with the LDS of LINES columns; its points stay in registers.
+template <int L, int M, int T, int LINES, bool LFAST, int S, int G, int ST>
+__device__ __forceinline__ void rstages_groups_from(float2 (&v)[G][M], float2* lds, int stride,
+                                                    const float2* __restrict__ tw, int tid) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) rstage_compute<L, M, T, LINES, LFAST, S, ST>(v[g], tw, tid);
+  if constexpr (ST + 1 < RPlan<L, M>::stages) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) rexchange<L, M, T, LINES, LFAST, ST>(v[g], lds, stride, tid);
+    rstages_groups_from<L, M, T, LINES, LFAST, S, G, ST + 1>(v, lds, stride, tw, tid);
+  }
+}
+
+template <int L, int M, int T, int LINES, bool LFAST, int S, int G>
+__device__ __forceinline__ void rfft_groups(float2 (&v)[G][M], float2* lds, int stride, const float2* __restrict__ tw,
+                                            int tid) {
+  rstages_groups_from<L, M, T, LINES, LFAST, S, G, 0>(v, lds, stride, tw, tid);
+}
+
 // Stage-0 input positions: for u < M/R0, r < R0: (line, j + r·L/R0) → v[u·R0 + r].
 template <int L, int M, int T, int LINES, bool LFAST, typename F>
 __device__ __forceinline__ void rload(float2 (&v)[M], int tid, F&& f) {

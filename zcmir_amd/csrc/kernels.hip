@@ -451,6 +451,44 @@ __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx
   });
 }
 
+// 2·C columns per workgroup in two groups of C sharing one LDS buffer (see rfft_groups).
+template <int L, int S, int C>
+__global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft_g2(float2* data, int nx, int planes, int rows_in,
+                                                               float scale, const float2* __restrict__ tw) {
+  constexpr int T = ColT<L, C>::T, M = ColT<L, C>::M, STR = ColT<L, C>::STR;
+  __shared__ float2 lds[C * STR];
+  const int tid = threadIdx.x;
+  const int groups = nx / (2 * C);
+  const int lb = xcd_remap((int)blockIdx.x, groups * planes);
+  const int plane = lb / groups, x0 = (lb - plane * groups) * 2 * C;
+  float2* base = data + (int64_t)plane * nx * L + x0;
+  float2 v[2][M];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+    rload<L, M, T, C, true>(v[g], tid, [&](int c, int y) {
+      return y < rows_in ? base[(int64_t)y * nx + g * C + c] : make_float2(0.0f, 0.0f);
+    });
+  rfft_groups<L, M, T, C, true, S, 2>(v, lds, STR, tw, tid);
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+    rstore<L, M, T, C, true>(v[g], tid, [&](int c, int y, float2 val) {
+      base[(int64_t)y * nx + g * C + c] = make_float2(val.x * scale, val.y * scale);
+    });
+}
+
+template <int L, int S, int C>
+static bool colfft_g2_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw,
+                         hipStream_t st) {
+  if constexpr (!ColT<L, C>::fits) {
+    return false;
+  } else {
+    if (nx % (2 * C)) return false;
+    const unsigned g = (unsigned)(nx / (2 * C) * planes);
+    k_colfft_g2<L, S, C><<<g, ColT<L, C>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw);
+    return true;
+  }
+}
+
 template <int L, int S, int C>
 static bool colfft_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st) {
   if constexpr (!ColT<L, C>::fits) {
@@ -472,6 +510,10 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     case 4: return colfft_LC<L, S, 4>(data, planes, nx, rows_in, scale, tw, st);
     case 8: return colfft_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st);
     case 16: return colfft_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st);
+    // two groups sharing the LDS: 16 = 2 x 8, 32 = 2 x 16 columns
+    case 17: return colfft_g2_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st);
+    case 33: return colfft_g2_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st);
+    case 9: return colfft_g2_LC<L, S, 4>(data, planes, nx, rows_in, scale, tw, st);
     default: return false;
   }
 }
