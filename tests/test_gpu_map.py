@@ -162,7 +162,7 @@ def test_fft2_every_column_tiling(reg_factory, n):
     reg = reg_factory()
     ran = 0
     try:
-        for cols in (1, 2, 4, 8, 16, 9, 17, 33):
+        for cols in (1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80):
             lib.zc_fft_tune(0, cols)
             try:
                 y = reg.fft2(x.copy())

@@ -443,11 +443,11 @@ __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx
   float2* base = data + (int64_t)plane * nx * L + x0;
   float2 v[M];
   rload<L, M, T, C, true>(v, tid, [&](int c, int y) {
-    return y < rows_in ? base[(int64_t)y * nx + c] : make_float2(0.0f, 0.0f);
+    return y < rows_in ? base[(unsigned)(y * nx + c)] : make_float2(0.0f, 0.0f);
   });
   rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
   rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
-    base[(int64_t)y * nx + c] = make_float2(val.x * scale, val.y * scale);
+    base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
   });
 }
 
@@ -466,13 +466,13 @@ __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft_g2(float2* data, int
 #pragma unroll
   for (int g = 0; g < 2; ++g)
     rload<L, M, T, C, true>(v[g], tid, [&](int c, int y) {
-      return y < rows_in ? base[(int64_t)y * nx + g * C + c] : make_float2(0.0f, 0.0f);
+      return y < rows_in ? base[(unsigned)(y * nx + g * C + c)] : make_float2(0.0f, 0.0f);
     });
   rfft_groups<L, M, T, C, true, S, 2>(v, lds, STR, tw, tid);
 #pragma unroll
   for (int g = 0; g < 2; ++g)
     rstore<L, M, T, C, true>(v[g], tid, [&](int c, int y, float2 val) {
-      base[(int64_t)y * nx + g * C + c] = make_float2(val.x * scale, val.y * scale);
+      base[(unsigned)(y * nx + g * C + c)] = make_float2(val.x * scale, val.y * scale);
     });
 }
 
@@ -485,6 +485,58 @@ static bool colfft_g2_LC(float2* data, int planes, int nx, int rows_in, float sc
     if (nx % (2 * C)) return false;
     const unsigned g = (unsigned)(nx / (2 * C) * planes);
     k_colfft_g2<L, S, C><<<g, ColT<L, C>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw);
+    return true;
+  }
+}
+
+// Streaming column FFT: one workgroup per C-column group runs the FFTs of that group in every
+// plane (planes [0, pa) of A, then [0, pb) of B), loading plane i+1 into registers while it
+// transforms plane i, so HBM traffic overlaps the LDS exchanges of a 1-workgroup-per-CU kernel.
+template <int L, int S, int C>
+__global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft_stream(float2* A, int pa, float2* B, int pb, int nx,
+                                                                   int rows_in, float scale,
+                                                                   const float2* __restrict__ tw) {
+  constexpr int T = ColT<L, C>::T, M = ColT<L, C>::M, STR = ColT<L, C>::STR;
+  __shared__ float2 lds[C * STR];
+  const int tid = threadIdx.x;
+  const int groups = nx / C;
+  const int x0 = xcd_remap((int)blockIdx.x, groups) * C;
+  const int64_t plane_sz = (int64_t)nx * L;
+  const int total = pa + pb;
+  auto base_of = [&](int i) { return (i < pa ? A + (int64_t)i * plane_sz : B + (int64_t)(i - pa) * plane_sz) + x0; };
+  float2 v[M], nxt[M];
+  {
+    const float2* b0 = base_of(0);
+    rload<L, M, T, C, true>(v, tid, [&](int c, int y) {
+      return y < rows_in ? b0[(unsigned)(y * nx + c)] : make_float2(0.0f, 0.0f);
+    });
+  }
+  for (int i = 0; i < total; ++i) {
+    if (i + 1 < total) {
+      const float2* bn = base_of(i + 1);
+      rload<L, M, T, C, true>(nxt, tid, [&](int c, int y) {
+        return y < rows_in ? bn[(unsigned)(y * nx + c)] : make_float2(0.0f, 0.0f);
+      });
+    }
+    rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
+    float2* bi = base_of(i);
+    rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+      bi[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
+    });
+    __syncthreads();   // the next plane's first exchange reuses the LDS
+#pragma unroll
+    for (int k = 0; k < M; ++k) v[k] = nxt[k];
+  }
+}
+
+template <int L, int S, int C>
+static bool colfft_stream_LC(float2* A, int pa, float2* B, int pb, int nx, int rows_in, float scale,
+                             const float2* tw, hipStream_t st) {
+  if constexpr (!ColT<L, C>::fits) {
+    return false;
+  } else {
+    if (nx % C) return false;
+    k_colfft_stream<L, S, C><<<(unsigned)(nx / C), ColT<L, C>::T, 0, st>>>(A, pa, B, pb, nx, rows_in, scale, tw);
     return true;
   }
 }
@@ -514,6 +566,10 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     case 17: return colfft_g2_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st);
     case 33: return colfft_g2_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st);
     case 9: return colfft_g2_LC<L, S, 4>(data, planes, nx, rows_in, scale, tw, st);
+    // streaming over planes (one workgroup per column group): 64 + columns
+    case 68: return colfft_stream_LC<L, S, 4>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
+    case 72: return colfft_stream_LC<L, S, 8>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
+    case 80: return colfft_stream_LC<L, S, 16>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
     default: return false;
   }
 }
