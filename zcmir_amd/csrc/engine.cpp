@@ -333,8 +333,8 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   n0buf.ensure(64);
   map.ensure((size_t)N * 4);
   const int nbands = (ny + rband - 1) / rband;
-  int nparts = 0;   // argmax partials, packed band after band
-  for (int b = 0; b < nbands; ++b) nparts += combine_blocks((int64_t)std::min(rband, ny - b * rband) * nx);
+  const int bparts = combine_blocks((int64_t)rband * nx);
+  const int nparts = bparts * nbands;
   arg_part.ensure((size_t)nparts * 8);
   res.ensure(16);
   // canvas stacks
@@ -384,12 +384,11 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   if (nbands == 1) {
     LAUNCH("combine", launch_combine(plan, corr.as<float>(), N, map.as<float>(), 0, N, n0, ca, arg_part.as<float>(), st));
   } else {
-    for (int b = 0, off = 0; b < nbands; ++b) {
+    for (int b = 0; b < nbands; ++b) {
       const int y0 = b * rband, yc = std::min(rband, ny - y0);
       LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st, 0, pi.n_out, y0, yc, rband, n0));
       LAUNCH("combine", launch_combine(plan, corr.as<float>(), (int64_t)rband * nx, map.as<float>(), (int64_t)y0 * nx,
-                                       (int64_t)yc * nx, n0, ca, arg_part.as<float>() + 2 * (size_t)off, st));
-      off += combine_blocks((int64_t)yc * nx);
+                                       (int64_t)yc * nx, n0, ca, arg_part.as<float>() + 2 * (size_t)b * bparts, st));
     }
     if (!ok) throw std::runtime_error("UnsupportedFftLength");
   }
