@@ -23,15 +23,14 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 
 def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
-    """Algorithmic HBM bytes per map of each map kernel (the minimal traffic of its role; a kernel
-    launched several times per map — per side, per band — shares these bytes over its launches)."""
+    """Algorithmic HBM bytes per launch of each map kernel (minimal traffic of its own role)."""
     N = nx * ny
     P, Q, C = plan["n_cplx"], plan["n_out"], plan["n_corr"]
     return {
         "warp_stack": 20 * S + 20 * canvas_px,
         "paste_stack": 20 * S + 20 * canvas_px,
-        "pack_rowfft": 2 * (20 * canvas_px + P * ch * nx * 8),
-        "colfft_fwd": 2 * (P * ch * nx * 8 + P * N * 8),
+        "pack_rowfft": 20 * canvas_px + P * ch * nx * 8,
+        "colfft_fwd": P * ch * nx * 8 + P * N * 8,
         "cmul": 2 * P * N * 8 + Q * N * 8,
         "colfft_inv": 2 * Q * N * 8,
         "rowifft_extract": Q * N * 8 + C * N * 4,
@@ -68,8 +67,6 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a copy kernel of this many MiB and exit")
     ap.add_argument("--fft-cols", type=int, default=0, help="columns per workgroup of the column FFT (0: default)")
-    ap.add_argument("--plane-band", type=int, default=-1, help="complex outputs per inverse band (-1: default)")
-    ap.add_argument("--row-band", type=int, default=-1, help="map rows per row-IFFT + combine band (-1: default)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -89,8 +86,6 @@ def main():
     three = a.corr == 28
     if a.fft_cols:
         zcmir_amd._load().zc_fft_tune(0, a.fft_cols)
-    if a.plane_band >= 0 or a.row_band >= 0:
-        zcmir_amd._load().zc_map_tune(max(a.plane_band, 0), max(a.row_band, 0))
     size, shift = (1024, 1024), (113, -57)
     mv, fx, _ = synth.shift_pair(size, shift, seed=11, quantize=True)
     reg = zcmir_amd.Registrar(half=False, symmetric=False, smi_features="three" if three else "laplacian")
@@ -121,10 +116,9 @@ def main():
     for name, (cnt, tot, avg) in prof.items():
         per_map = cnt / a.steps
         b = kb.get(name)
-        per_launch = b / per_map if b else None
         kernels[name] = {
             "launches_per_map": per_map, "avg_ms": avg, "ms_per_map": tot / a.steps,
-            "alg_bytes": per_launch, "gbs": (per_launch / (avg * 1e-3) / 1e9) if b else None,
+            "alg_bytes": b, "gbs": (b / (avg * 1e-3) / 1e9) if b else None,
         }
     dom = max((k for k in kernels if kernels[k]["alg_bytes"]), key=lambda k: kernels[k]["ms_per_map"])
     dk = kernels[dom]
@@ -136,7 +130,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(dk["gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": round(dk["avg_ms"], 5)}
-    total_alg = sum(kb[k] for k in kb if k in kernels)
+    total_alg = sum(kb[k] * kernels[k]["launches_per_map"] for k in kb if k in kernels)
     model_bytes = 40 * size[0] * size[1] + 64 * plan["n_cplx"] * sm.n * sm.ny + 16 * plan["n_out"] * sm.n * sm.ny \
         + 4 * sm.n * sm.ny
 

@@ -174,22 +174,3 @@ def test_fft2_every_column_tiling(reg_factory, n):
     finally:
         lib.zc_fft_tune(0, 0)
     assert ran >= 2
-
-
-@pytest.mark.parametrize("plane_band,row_band", [(4, 0), (1, 0), (0, 256), (0, 100)])
-def test_shift_map_banded_identical(reg_factory, plane_band, row_band):
-    """Banding the inverse transform (zc_map_tune) changes the schedule, not the arithmetic."""
-    import zcmir_amd
-
-    lib = zcmir_amd._load()
-    mv, fx, _ = _pair((300, 260), (9, 4), seed=12)
-    reg = reg_factory(half=False)
-    reg.set_images(mv, fx, preprocess=False)
-    ref = reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=1024)
-    try:
-        lib.zc_map_tune(plane_band, row_band)
-        got = reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=1024)
-    finally:
-        lib.zc_map_tune(0, 0)
-    assert np.array_equal(got.map, ref.map)
-    assert (got.dx, got.dy, got.peak) == (ref.dx, ref.dy, ref.peak)

@@ -124,7 +124,6 @@ _SIGS = {
     # this library's own hooks (no reference counterpart)
     "zc_rank": ([P, U32, f32p, SZ, u32p, u32p, f32p], I32),
     "zc_bench_shift_map": ([P, f64p, U32, U32, U32, f64p], I32),
-    "zc_map_tune": ([U32, U32], None),
 }
 
 LIB_NAME = "libzcsmi.so"

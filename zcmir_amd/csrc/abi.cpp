@@ -289,13 +289,6 @@ int32_t zc_rank(void* p, uint32_t side, float* rank, size_t len, uint32_t* hist,
   });
 }
 
-// Banding of the map's inverse transform: complex outputs per band of column IFFT + row IFFT,
-// or map rows per band of row IFFT + combine (0: unbanded); process-wide.
-void zc_map_tune(uint32_t plane_band, uint32_t row_band) {
-  zc::MapTune::plane_band = (int)plane_band;
-  zc::MapTune::row_band = (int)row_band;
-}
-
 // `iters` shift maps of H back to back, no readback between them; the mean time per map (ms,
 // HIP events on the engine's stream) into ms_per_map.
 int32_t zc_bench_shift_map(void* p, const double* H, uint32_t flags, uint32_t cap, uint32_t iters,

@@ -100,7 +100,7 @@ Engine::~Engine() {
     s.stack.release();
   }
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
-                  &tmp_u32, &tmp_i64, &n0buf})
+                  &tmp_u32, &tmp_i64})
     b->release();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
@@ -324,17 +324,9 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   spec_a.ensure((size_t)N * 8 * pi.n_cplx);
   spec_b.ensure((size_t)N * 8 * pi.n_cplx);
   work.ensure((size_t)N * 8 * pi.n_out);
-  // banding of the inverse (MapTune): plane bands run colfft_inv + row IFFT per band of
-  // complex outputs; row bands run row IFFT + combine per band of map rows (the corr buffer then
-  // holds one band)
-  const int pband = MapTune::plane_band > 0 ? std::min<int>(MapTune::plane_band, pi.n_out) : 0;
-  const int rband = (pband == 0 && MapTune::row_band > 0) ? std::min<int>(MapTune::row_band, ny) : ny;
-  corr.ensure((size_t)rband * nx * 4 * pi.n_corr);
-  n0buf.ensure(64);
+  corr.ensure((size_t)N * 4 * pi.n_corr);
   map.ensure((size_t)N * 4);
-  const int nbands = (ny + rband - 1) / rband;
-  const int bparts = combine_blocks((int64_t)rband * nx);
-  const int nparts = bparts * nbands;
+  const int nparts = combine_blocks(N);
   arg_part.ensure((size_t)nparts * 8);
   res.ensure(16);
   // canvas stacks
@@ -359,18 +351,8 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
   LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
   LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
-  float* n0 = n0buf.as<float>();
-  if (pband > 0) {
-    for (int q0 = 0; q0 < pi.n_out; q0 += pband) {
-      const int nq = std::min(pband, pi.n_out - q0);
-      LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>() + (size_t)q0 * N, nq, nx, ny, ny, true, y_tw, st, FftTune::cols));
-      LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st, q0, nq, 0, ny, ny, n0));
-    }
-  } else {
-    LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, FftTune::cols));
-    if (nbands == 1)
-      LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st, 0, pi.n_out, 0, ny, ny, n0));
-  }
+  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, FftTune::cols));
+  LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   const uint32_t floor_n = min_overlap > 0 ? (uint32_t)std::floor(min_overlap * (double)cw * (double)ch) : 0u;
   CombineArgs ca{};
@@ -381,17 +363,7 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   ca.ridge = kRidge;
   const double gcell = (double)W / (double)cw;
   ca.area = std::max(1.0, pair_area() / (gcell * gcell));
-  if (nbands == 1) {
-    LAUNCH("combine", launch_combine(plan, corr.as<float>(), N, map.as<float>(), 0, N, n0, ca, arg_part.as<float>(), st));
-  } else {
-    for (int b = 0; b < nbands; ++b) {
-      const int y0 = b * rband, yc = std::min(rband, ny - y0);
-      LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st, 0, pi.n_out, y0, yc, rband, n0));
-      LAUNCH("combine", launch_combine(plan, corr.as<float>(), (int64_t)rband * nx, map.as<float>(), (int64_t)y0 * nx,
-                                       (int64_t)yc * nx, n0, ca, arg_part.as<float>() + 2 * (size_t)b * bparts, st));
-    }
-    if (!ok) throw std::runtime_error("UnsupportedFftLength");
-  }
+  LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
   ++counts[2];

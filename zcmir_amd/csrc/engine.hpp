@@ -127,12 +127,6 @@ struct FftTune {
   static inline std::atomic<int> rows{0}, cols{0};
 };
 
-// Banding of the inverse transform (zc_map_tune): complex outputs per band of column IFFT +
-// row IFFT, or map rows per band of row IFFT + combine; 0 = unbanded.
-struct MapTune {
-  static inline std::atomic<int> plane_band{0}, row_band{0};
-};
-
 struct MapOut {
   Frame fr;
   GridPlan g;
@@ -183,7 +177,7 @@ class Engine {
     ZC_HIP(hipGetLastError());
   }
 
-  DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64, n0buf;
+  DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
   void ensure_acf(int s);

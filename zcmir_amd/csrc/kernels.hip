@@ -625,35 +625,31 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
 }
 
 // ── row IFFT + extract ────────────────────────────────────────────────────────────────────
-// Inverse row FFT of complex output q = q0 + blockIdx.y (rows y_begin .. y_begin + y_count of
-// an L × ny plane) → correlation planes 2q (re) and 2q+1 (im) of a corr buffer holding
-// corr_rows rows per plane (row y at y - y_begin). The zero-lag overlap n(0) goes to n0.
+// Inverse row FFT of complex output q (rows of L) → correlation planes 2q (re) and 2q+1 (im).
 template <int L, int PID>
 __global__ __launch_bounds__(RowT<L>::T) void k_rowifft_extract(const float2* __restrict__ work, float* corr, int ny,
-                                                                int q0, int y_begin, int y_count, int corr_rows,
-                                                                float* n0, const float2* __restrict__ tw) {
+                                                                const float2* __restrict__ tw) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
   __shared__ float2 lds[LINES * STR];
-  const int tid = threadIdx.x, q = q0 + (int)blockIdx.y;
-  const int r0 = blockIdx.x * LINES;   // band-relative row of line 0
-  const int64_t N = (int64_t)L * ny, NB = (int64_t)L * corr_rows;
-  const float2* src = work + q * N + (int64_t)y_begin * L;
+  const int tid = threadIdx.x, q = blockIdx.y;
+  const int y0 = blockIdx.x * LINES;
+  const int64_t N = (int64_t)L * ny;
+  const float2* src = work + q * N;
   float2 v[M];
   rload<L, M, T, LINES, false>(v, tid, [&](int line, int x) {
-    const int r = r0 + line;
-    return r < y_count ? src[(unsigned)(r * L + x)] : make_float2(0.0f, 0.0f);
+    const int y = y0 + line;
+    return y < ny ? src[(int64_t)y * L + x] : make_float2(0.0f, 0.0f);
   });
   rfft<L, M, T, LINES, false, 1>(v, lds, STR, tw, tid);
-  float* o1 = corr + (int64_t)(2 * q) * NB;
-  float* o2 = corr + (int64_t)(2 * q + 1) * NB;
+  float* o1 = corr + (int64_t)(2 * q) * N;
+  float* o2 = corr + (int64_t)(2 * q + 1) * N;
   const bool two = 2 * q + 1 < P.n_corr;
   rstore<L, M, T, LINES, false>(v, tid, [&](int line, int x, float2 val) {
-    const int r = r0 + line;
-    if (r < y_count) {
-      o1[(unsigned)(r * L + x)] = val.x;
-      if (two) o2[(unsigned)(r * L + x)] = val.y;
-      if (q == 0 && x == 0 && r + y_begin == 0) *n0 = val.x;
+    const int y = y0 + line;
+    if (y < ny) {
+      o1[(int64_t)y * L + x] = val.x;
+      if (two) o2[(int64_t)y * L + x] = val.y;
     }
   });
 }
@@ -693,22 +689,19 @@ __device__ __forceinline__ bool chol(double (&g)[4][4], double (&rd)[4]) {
   return true;
 }
 
-// Lags t_begin .. t_begin + count of the map; corr holds `count` lags per plane (plane stride
-// `stride`), lag t at t - t_begin; n0: the zero-lag overlap.
 template <int PID>
-__global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr, int64_t stride, float* map,
-                                                 int64_t t_begin, int64_t count, const float* __restrict__ n0p,
+__global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr, float* map, int64_t nn,
                                                  CombineArgs a, float* arg_part) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr bool EXACT = PID & 1;
   constexpr int NF = (PID & 2) ? 3 : 4;
-  auto C = [&](int num, int64_t i) -> double { return (double)corr[(int64_t)P.kept_of[num] * stride + i]; };
-  const double n0 = (double)*n0p;
+  auto C = [&](int num, int64_t t) -> double { return (double)corr[(int64_t)P.kept_of[num] * nn + t]; };
+  const double n0 = C(0, 0);
   const bool calib = EXACT && a.calib;
   const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
   float bv = -3.4e38f;
   uint32_t bi = 0xffffffffu;
-  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < count; t += (int64_t)gridDim.x * 256) {
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
     const double n = C(0, t);
     const double neff = n / fmax(a.area, 1.0);
     double score = 0.0;
@@ -787,10 +780,10 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
       if (!(score == score) || isinf(score)) score = 0.0;
     }
     const float sv = (float)score;
-    map[t_begin + t] = sv;
-    if (better(sv, (uint32_t)(t_begin + t), bv, bi)) {
+    map[t] = sv;
+    if (better(sv, (uint32_t)t, bv, bi)) {
       bv = sv;
-      bi = (uint32_t)(t_begin + t);
+      bi = (uint32_t)t;
     }
   }
   // block argmax
@@ -1027,20 +1020,19 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
 }
 
 template <int L>
-static void rowifft_L(int plan, const float2* work, float* corr, int ny, int q0, int nq, int y_begin, int y_count,
-                      int corr_rows, float* n0, const float2* tw, hipStream_t st) {
-  dim3 g((y_count + RowT<L>::LINES - 1) / RowT<L>::LINES, nq);
+static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st) {
+  dim3 g((ny + RowT<L>::LINES - 1) / RowT<L>::LINES, kPlans[plan].n_out);
   switch (plan) {
-    case 0: k_rowifft_extract<L, 0><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, q0, y_begin, y_count, corr_rows, n0, tw); break;
-    case 1: k_rowifft_extract<L, 1><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, q0, y_begin, y_count, corr_rows, n0, tw); break;
-    case 2: k_rowifft_extract<L, 2><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, q0, y_begin, y_count, corr_rows, n0, tw); break;
-    default: k_rowifft_extract<L, 3><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, q0, y_begin, y_count, corr_rows, n0, tw); break;
+    case 0: k_rowifft_extract<L, 0><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 1: k_rowifft_extract<L, 1><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 2: k_rowifft_extract<L, 2><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    default: k_rowifft_extract<L, 3><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
   }
 }
 
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
-                            hipStream_t st, int q0, int nq, int y_begin, int y_count, int corr_rows, float* n0) {
-#define CALL(LL) rowifft_L<LL>(plan, work, corr, ny, q0, nq, y_begin, y_count, corr_rows, n0, tw_x, st)
+                            hipStream_t st) {
+#define CALL(LL) rowifft_L<LL>(plan, work, corr, ny, tw_x, st)
   ZC_POW2_SWITCH(nx, CALL)
 #undef CALL
   return true;
@@ -1048,14 +1040,14 @@ bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, i
 
 int combine_blocks(int64_t nn) { return (int)std::min<int64_t>((nn + 255) / 256, 4096); }
 
-void launch_combine(int plan, const float* corr, int64_t stride, float* map, int64_t t_begin, int64_t count,
-                    const float* n0, const CombineArgs& a, float* arg_part, hipStream_t st) {
-  const int g = combine_blocks(count);
+void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
+                    hipStream_t st) {
+  const int g = combine_blocks(nn);
   switch (plan) {
-    case 0: k_combine<0><<<g, 256, 0, st>>>(corr, stride, map, t_begin, count, n0, a, arg_part); break;
-    case 1: k_combine<1><<<g, 256, 0, st>>>(corr, stride, map, t_begin, count, n0, a, arg_part); break;
-    case 2: k_combine<2><<<g, 256, 0, st>>>(corr, stride, map, t_begin, count, n0, a, arg_part); break;
-    default: k_combine<3><<<g, 256, 0, st>>>(corr, stride, map, t_begin, count, n0, a, arg_part); break;
+    case 0: k_combine<0><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 1: k_combine<1><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 2: k_combine<2><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    default: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
   }
 }
 

@@ -48,20 +48,16 @@ bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool i
                    hipStream_t st, int lines = 0);
 void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* work, int nx, int ny,
                  hipStream_t st);
-// inverse row FFTs of complex outputs q0 .. q0 + nq, rows y_begin .. y_begin + y_count, into a
-// corr buffer of corr_rows rows per plane; n0 receives the zero-lag overlap
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
-                            hipStream_t st, int q0, int nq, int y_begin, int y_count, int corr_rows, float* n0);
+                            hipStream_t st);
 struct CombineArgs {
   int exact, calib, nf;
   float min_n;
   double area, ridge;
 };
 int combine_blocks(int64_t nn);
-// per-lag score of lags t_begin .. t_begin + count (corr: `count` lags per plane, plane stride
-// `stride`); block argmax partials into arg_part (combine_blocks(count) pairs)
-void launch_combine(int plan, const float* corr, int64_t stride, float* map, int64_t t_begin, int64_t count,
-                    const float* n0, const CombineArgs& a, float* arg_part, hipStream_t st);
+void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
+                    hipStream_t st);
 void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st);
 
 // plain 2-D FFT of `planes` n × n planes (tests, benchmarks)
