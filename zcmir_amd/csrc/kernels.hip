@@ -426,9 +426,10 @@ struct ColT {
 };
 
 // lines (columns) per workgroup of the column FFT: the default, or a tuned value that fits
-inline int col_lines_default(int L) { return L >= 4096 ? 2 : (L >= 2048 ? 4 : (L >= 1024 ? 8 : 16)); }
+// the widest row segments whose LDS fits (measured best at 2048: 8 columns, 1 workgroup per CU)
+inline int col_lines_default(int L) { return L >= 4096 ? 4 : (L >= 2048 ? 8 : 16); }
 static_assert(ColT<32, 16>::fits && ColT<64, 16>::fits && ColT<128, 16>::fits && ColT<256, 16>::fits &&
-                  ColT<512, 16>::fits && ColT<1024, 8>::fits && ColT<2048, 4>::fits && ColT<4096, 2>::fits,
+                  ColT<512, 16>::fits && ColT<1024, 16>::fits && ColT<2048, 8>::fits && ColT<4096, 4>::fits,
               "every default column tiling must be launchable");
 
 template <int L, int S, int C>
