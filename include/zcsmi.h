@@ -147,6 +147,30 @@ int32_t zc_rank(void* e, uint32_t side, float* rank, size_t len, uint32_t* hist,
 int32_t zc_bench_shift_map(void* e, const double H[9], uint32_t flags, uint32_t cap, uint32_t iters,
                            double* ms_per_map);
 
+/* the rotation x scale sweep alone (the first stage of zc_search, search.zig:300-494 Sweep.sweep,
+ * with the candidate grid and options of the current settings; sw_full = 1 extends sw_grid to
+ * one canvas cell per fixed pixel) */
+typedef struct {
+  uint32_t nc, n, grid, n_peaks; /* candidates, largest plane length, canvas cells, raw peaks */
+  double cell, D, ox, oy;        /* cell size and canvas side (fixed pixels), canvas origin */
+} zc_sweep_info;
+typedef struct {                 /* search.zig:145 Peak */
+  double score;                  /* n in fixed pixels */
+  uint32_t cand, it, is, ik, ia; /* candidate (theta-major), theta / sigma / shear / stretch index */
+  uint32_t _pad;
+  double th, s;
+  int32_t sx, sy;                /* canvas shift in cells */
+  double H[9];                   /* corrected pose, moving -> fixed */
+} zc_sweep_peak;
+/* H0: prior pose (NULL: the settings' rule). Writes up to cap raw peaks (every candidate's top 4
+ * local maxima above the floor) and nbest per-candidate best scores; returns the peak count. */
+int32_t zc_sweep(void* e, const double H0[9], zc_sweep_info* info, zc_sweep_peak* peaks, size_t cap, float* best,
+                 size_t nbest);
+/* the score map of the last sweep batch's first candidate; dims = (nx, ny) */
+int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
+/* candidates per sweep batch (0: 64); process-wide */
+void zc_sweep_tune(uint32_t batch);
+
 /* ── reference entry points outside the SMI hot path (exports.zig:95-138, 232, 245, 284-376,
  *    382, 397, 430, 458-489): exported so the reference's full ctypes symbol list binds; each
  *    fails with zc_error "Unsupported" (or is a no-op tuning hook) ── */

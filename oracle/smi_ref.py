@@ -455,11 +455,12 @@ def f_to_z(f, d1, d2):
 
 
 def combine(corr: np.ndarray, exact: bool = True, calibrated: bool = False, nf: int = 4,
-            min_n: float = MIN_N, ridge: float = RIDGE, area: float = 1.0) -> np.ndarray:
+            min_n: float = MIN_N, ridge: float = RIDGE, area: float = 1.0, zero_n: float | None = None) -> np.ndarray:
     """Per-lag SMI from the correlation planes, float64 (smi.wgsl:825-931 combine_cell).
 
     corr: (45, ...) (or 25 for global) indexed by correlation number; entries of the fourth
-    feature unused when nf == 3. Non-calibrated floor: n < max(min_n, 0.08 n(lag 0)) → 0;
+    feature unused when nf == 3. Non-calibrated floor: n < max(min_n, 0.08 n0) → 0, n0 = n at
+    lag 0 (or `zero_n`: the sweep's zero-displacement lag, sweep.wgsl:247);
     calibrated: n < min_n or n / area <= 10 → 0.
     """
     corr = np.asarray(corr, np.float64)
@@ -468,7 +469,8 @@ def combine(corr: np.ndarray, exact: bool = True, calibrated: bool = False, nf: 
     n = c[0]
     calib = exact and calibrated
     neff = n / max(area, 1.0)
-    floor_n = max(float(min_n), 0.08 * max(float(c[0, 0]), 0.0))
+    n0 = float(c[0, 0]) if zero_n is None else float(zero_n)
+    floor_n = max(float(min_n), 0.08 * max(n0, 0.0))
     if calib:
         floor_n = float(min_n)
     ok = ~(n < floor_n)
@@ -689,3 +691,236 @@ def pose_score(feat_a, feat_b, H, exact: bool = True, symmetric: bool = False):
         return fwd, fwd, None
     inv = score_from_moments(pose_moments(feat_b, feat_a, np.linalg.inv(np.asarray(H, np.float64).reshape(3, 3))), exact)
     return 0.5 * (fwd + inv), fwd, inv
+
+
+# ── rotation × scale sweep (search.zig:78-143, 300-529; sweep.wgsl:64-371) ──────────────────
+def sweep_grid(rot_lo, rot_hi, n_theta, s_min, s_max, n_sigma, shear_max=0.0, n_shear=1, aniso_max=0.0,
+               n_aniso=1) -> dict:
+    """Candidate axes (search.zig:89-116 sweepGrid): the full circle without repeating ±180°, a
+    partial range with both ends; scales and stretches log-spaced, shears linear about 0."""
+    def linspace(lo, hi, n):
+        return [0.5 * (lo + hi)] if n == 1 else [lo + (i * (hi - lo)) / (n - 1) for i in range(n)]
+    nT = max(1, n_theta)
+    lo_d, span = min(rot_lo, rot_hi), abs(rot_hi - rot_lo)
+    wrap = span >= 360 - 1e-9
+    th = [(-180 + 360 * i / nT if wrap else (lo_d + span / 2 if nT == 1 else lo_d + span * i / (nT - 1)))
+          * math.pi / 180 for i in range(nT)]
+    llo = math.log(max(1e-3, min(s_min, s_max)))
+    lhi = math.log(max(1e-3, max(s_min, s_max)))
+    sg = [math.exp(v) for v in linspace(llo, lhi, n_sigma)]
+    kmax = abs(shear_max)
+    sh = linspace(-kmax, kmax, n_shear if kmax > 0 else 1)
+    amax = math.log(1 + abs(aniso_max))
+    an = [math.exp(v) for v in linspace(-amax, amax, n_aniso if amax > 0 else 1)]
+    return dict(thetas=th, sigmas=sg, shears=sh, anisos=an, wrap=wrap)
+
+
+def cand_linear(th, s, k=0.0, a=1.0):
+    """σ R(θ) diag(a, 1/a) [1 k; 0 1] row-major (search.zig:119-127)."""
+    c, sn = s * math.cos(th), s * math.sin(th)
+    return [c * a, c * a * k - sn / a, sn * a, sn * a * k + c / a]
+
+
+def about_c(c, L):
+    """X ↦ c + L (X − c) (search.zig:130-132)."""
+    return np.array([[L[0], L[1], c[0] - L[0] * c[0] - L[1] * c[1]],
+                     [L[2], L[3], c[1] - L[2] * c[0] - L[3] * c[1]], [0.0, 0.0, 1.0]])
+
+
+def sv2(L):
+    """Singular values of a 2×2, largest first (search.zig:135-143); checked against np.linalg.svd."""
+    a = L[0] ** 2 + L[2] ** 2
+    b = L[0] * L[1] + L[2] * L[3]
+    d = L[1] ** 2 + L[3] ** 2
+    t = 0.5 * (a + d)
+    r = math.sqrt(max(0.0, 0.25 * (a - d) ** 2 + b * b))
+    return math.sqrt(t + r), math.sqrt(max(0.0, t - r))
+
+
+def map_pt(H, x, y):
+    """lie.mapPt (lie.zig:311-317): |Z| < 1e-8 divides by +1e-8."""
+    H = np.asarray(H, np.float64).reshape(3, 3)
+    X, Y, Z = H @ np.array([x, y, 1.0])
+    z = 1e-8 if abs(Z) < 1e-8 else Z
+    return X / z, Y / z
+
+
+def _cells(a, b, o0, c, g):
+    """Canvas cells [lo, hi) over dest-pixel extent [a, b], one cell of margin (search.zig:329-334)."""
+    lo = min(max(math.floor((a - o0) / c) - 1, 0), g)
+    hi = min(max(math.ceil((b - o0) / c) + 1, lo + 1), g)
+    return lo, hi
+
+
+def sweep_window(feat, Hi, gx0, gy0, win_x, win_y, grid, k, ox, oy, cell):
+    """(5, win_y, win_x) stack values [w, w f0..w f3] (w = 1) of a candidate's window: cell
+    (gx0 + x, gy0 + y) is the mean of k×k bilinear taps at Hi · tap point, each tap zero unless
+    all four corners lie inside the source (sweep.wgsl:64-85 tap, 106-145 pack_sweep); cells
+    outside the canvas are zero."""
+    _, sh, sw = feat.shape
+    Hi = np.asarray(Hi, np.float64).reshape(3, 3)
+    cy, cx = np.mgrid[0:win_y, 0:win_x]
+    cx = cx + gx0
+    cy = cy + gy0
+    inside = (cx < grid) & (cy < grid)
+    acc = np.zeros((5, win_y, win_x))
+    for ty in range(k):
+        py = oy + (cy + (ty + 0.5) / k) * cell
+        for tx in range(k):
+            px = ox + (cx + (tx + 0.5) / k) * cell
+            qx = Hi[0, 0] * px + Hi[0, 1] * py + Hi[0, 2]
+            qy = Hi[1, 0] * px + Hi[1, 1] * py + Hi[1, 2]
+            qz = Hi[2, 0] * px + Hi[2, 1] * py + Hi[2, 2]
+            okz = np.abs(qz) >= 1e-8
+            qz = np.where(okz, qz, 1.0)
+            x = qx / qz - 1.0
+            y = qy / qz - 1.0
+            ok = inside & okz & (x >= 0) & (y >= 0) & (x < sw - 1) & (y < sh - 1)
+            xs, ys = np.where(ok, x, 0.0), np.where(ok, y, 0.0)
+            x0, y0 = np.floor(xs).astype(np.int64), np.floor(ys).astype(np.int64)
+            fx, fy = xs - x0, ys - y0
+            w00, w10, w01, w11 = (1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy
+            acc[0] += np.where(ok, w00 + w10 + w01 + w11, 0.0)
+            for p in range(4):
+                f = feat[p]
+                v = w00 * f[y0, x0] + w10 * f[y0, x0 + 1] + w01 * f[y0 + 1, x0] + w11 * f[y0 + 1, x0 + 1]
+                acc[1 + p] += np.where(ok, v, 0.0)
+    return acc / (k * k)
+
+
+def sweep_peaks(m: np.ndarray, n_peak: int = 4):
+    """Top local maxima of a score map (sweep.wgsl:326-371 peaks_sweep): lag class c = t mod 256
+    keeps its largest value above 0 that no 3×3 periodic neighbour exceeds (first lag on ties);
+    the n_peak best classes, as (value, flat lag) sorted by value (lower lag on ties); empty
+    classes are (0, 0xffffffff)."""
+    ny, nx = m.shape
+    ok = m > 0
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dx or dy:
+                ok &= ~(np.roll(m, (-dy, -dx), axis=(0, 1)) > m)
+    flat = m.ravel()
+    okf = ok.ravel()
+    best = []
+    for c in range(256):
+        idx = np.arange(c, flat.size, 256)
+        idx = idx[okf[idx]]
+        if idx.size == 0:
+            best.append((0.0, 0xFFFFFFFF))
+            continue
+        v = flat[idx]
+        j = int(np.argmax(v))   # first maximum = lowest lag of this class
+        best.append((float(v[j]), int(idx[j])))
+    best.sort(key=lambda e: (-e[0], e[1]))
+    return best[:n_peak]
+
+
+def sweep(prep_m: dict, prep_f: dict, H0, gv: dict, grid: int = 128, exact: bool = True, n_peak: int = 4,
+          min_overlap: float = 0.05, full: bool = False, keep_maps: bool = False):
+    """Every candidate's shift map on the sweep canvas, reduced to its top local maxima
+    (search.zig:300-494 Sweep.sweep; sweep.wgsl pack_sweep / cmul_sweep / combine_sweep /
+    peaks_sweep). Plane lengths are the powers of two >= max(32, M + F − 1) (the layout the
+    peak classes are defined on); the values per lag do not depend on the length.
+
+    Returns dict(peaks=[...], best, maps (if keep_maps: per candidate the map, its n plane and
+    overlap floor), cell, grid, D, ox, oy, fc, cands, min_n)."""
+    feat_m, feat_f = prep_m["feat"], prep_f["feat"]
+    three = prep_m["mode"] == "three"
+    _, mh, mw = feat_m.shape
+    _, fh, fw = feat_f.shape
+    H0 = np.asarray(H0, np.float64).reshape(3, 3)
+    cm = map_pt(H0, (mw + 1) / 2, (mh + 1) / 2)
+    s_max = max(sv2(cand_linear(0.0, s, k, a))[0] for s in gv["sigmas"] for k in gv["shears"] for a in gv["anisos"])
+    D = min(2 * max(fw, fh), max(max(fw, fh), s_max * math.hypot(mw, mh)))
+    if full:
+        grid = int(math.ceil(D - 1e-9))
+    cell = D / grid
+    ox, oy = (fw + 1) / 2 - D / 2, (fh + 1) / 2 - D / 2
+    f32 = lambda v: float(np.float32(v))
+    oxf, oyf, cellf = f32(ox), f32(oy), f32(cell)
+    fcx, fcy = _cells(0.5, fw + 0.5, ox, cell, grid), _cells(0.5, fh + 0.5, oy, cell, grid)
+    Fx, Fy = fcx[1] - fcx[0], fcy[1] - fcy[0]
+    s0r = math.sqrt(abs(H0[0, 0] * H0[1, 1] - H0[0, 1] * H0[1, 0]))
+    s0 = 1.0 if (s0r == 0 or math.isnan(s0r)) else s0r
+    plen = lambda need: 1 << max(5, (max(need, 1) - 1).bit_length())
+    lst = corr_list(exact)
+    keep = kept_corrs(exact, three)
+    pairs = [lst[c] for c in keep]
+    min_n = max(MIN_N, js_round(min_overlap * (fw * fh) / (cell * cell)))
+    kf = int(min(6, max(1, math.ceil(cell - 1e-3))))
+    fixed_cache = {}
+    peaks, best, maps, cands = [], [], [], []
+    ci = 0
+    for it, th in enumerate(gv["thetas"]):
+        for is_, s in enumerate(gv["sigmas"]):
+            for ik, kk in enumerate(gv["shears"]):
+                for ia, a in enumerate(gv["anisos"]):
+                    L = cand_linear(th, s, kk, a)
+                    H = about_c(cm, L) @ H0
+                    pts = [map_pt(H, qx, qy) for qx, qy in ((0.5, 0.5), (mw + 0.5, 0.5), (mw + 0.5, mh + 0.5),
+                                                            (0.5, mh + 0.5))]
+                    xs, ys = [p[0] for p in pts], [p[1] for p in pts]
+                    if all(math.isfinite(v) for v in xs + ys):
+                        mcx = _cells(min(xs), max(xs), ox, cell, grid)
+                        mcy = _cells(min(ys), max(ys), oy, cell, grid)
+                    else:
+                        mcx = mcy = (0, grid)
+                    nx, ny = plen(mcx[1] - mcx[0] + Fx - 1), plen(mcy[1] - mcy[0] + Fy - 1)
+                    taps = int(min(6, max(1, math.ceil(cell / (sv2(L)[1] * s0) - 1e-3))))
+                    if (nx, ny) not in fixed_cache:
+                        sf = sweep_window(feat_f, np.eye(3), fcx[0], fcy[0], Fx, Fy, grid, kf, oxf, oyf, cellf)
+                        fixed_cache[(nx, ny)] = real_planes(sf, exact)
+                    pb = fixed_cache[(nx, ny)]
+                    Hi = np.linalg.inv(H).astype(np.float32).astype(np.float64)
+                    wx, wy = nx - Fx + 1, ny - Fy + 1
+                    sm = sweep_window(feat_m, Hi, mcx[0], mcy[0], wx, wy, grid, taps, oxf, oyf, cellf)
+                    pa = real_planes(sm, exact)
+                    ck = correlations_fft(pa, pb, nx, ny, pairs)
+                    corr = np.zeros((len(lst), ny, nx))
+                    corr[keep] = ck
+                    zl = ((fcy[0] - mcy[0]) % ny) * nx + ((fcx[0] - mcx[0]) % nx)
+                    n0 = corr[0].ravel()[zl]
+                    m = combine(corr, exact=exact, nf=3 if three else 4, min_n=min_n, zero_n=n0)
+                    if keep_maps:
+                        maps.append(dict(map=m, n=corr[0], floor=max(float(min_n), 0.08 * max(n0, 0.0))))
+                    cands.append(dict(it=it, is_=is_, ik=ik, ia=ia, th=th, s=s, H=H, gx0=mcx[0], gy0=mcy[0],
+                                      nx=nx, ny=ny, taps=taps))
+                    b = 0.0
+                    for k, (v, t) in enumerate(sweep_peaks(m, n_peak)):
+                        v *= cell * cell
+                        if not (v > 0) or t >= nx * ny:
+                            continue
+                        if k == 0:
+                            b = v
+                        py, px = divmod(t, nx)
+                        lx = px if px < wx else px - nx
+                        ly = py if py < wy else py - ny
+                        sx, sy = mcx[0] + lx - fcx[0], mcy[0] + ly - fcy[0]
+                        Hc = translate(-sx * cell, -sy * cell) @ H
+                        peaks.append(dict(score=v, cand=ci, it=it, is_=is_, ik=ik, ia=ia, th=th, s=s, sx=sx, sy=sy,
+                                          H=Hc, lag=t))
+                    best.append(b)
+                    ci += 1
+    return dict(peaks=peaks, best=np.array(best), maps=maps, cell=cell, grid=grid, D=D, ox=ox, oy=oy,
+                fc=(fcx, fcy), cands=cands, min_n=min_n)
+
+
+def nms4(peaks, n_theta: int, keep: int, wrap: bool):
+    """Non-maximum suppression over (θ, σ, shear, stretch, shift) (search.zig:500-529): strongest
+    first (stable), dropped within ±2 θ steps (circular when wrap), ±2 σ, ±1 shear and stretch
+    steps and 4 cells of a kept peak."""
+    kept = []
+    for p in sorted(peaks, key=lambda q: -q["score"]):
+        ok = True
+        for q in kept:
+            a = abs(p["it"] - q["it"])
+            dt = min(a, n_theta - a) if wrap else a
+            if (dt <= 2 and abs(p["is_"] - q["is_"]) <= 2 and abs(p["ik"] - q["ik"]) <= 1 and abs(p["ia"] - q["ia"]) <= 1
+                    and math.hypot(p["sx"] - q["sx"], p["sy"] - q["sy"]) <= 4):
+                ok = False
+                break
+        if ok:
+            kept.append(p)
+            if len(kept) >= keep:
+                break
+    return kept

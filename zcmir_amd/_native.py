@@ -59,6 +59,16 @@ class SearchResult(ctypes.Structure):
     _fields_ = [("H", Mat), ("score", F64), ("prev", F64), ("improved", U32), ("count_a", U32), ("count_b", U32), ("_pad", U32)]
 
 
+class SweepInfo(ctypes.Structure):
+    _fields_ = [("nc", U32), ("n", U32), ("grid", U32), ("n_peaks", U32), ("cell", F64), ("D", F64), ("ox", F64),
+                ("oy", F64)]
+
+
+class SweepPeak(ctypes.Structure):
+    _fields_ = [("score", F64), ("cand", U32), ("it", U32), ("is_", U32), ("ik", U32), ("ia", U32), ("_pad", U32),
+                ("th", F64), ("s", F64), ("sx", I32), ("sy", I32), ("H", Mat)]
+
+
 class Canvas(ctypes.Structure):
     _fields_ = [("ox", I32), ("oy", I32), ("ow", U32), ("oh", U32), ("Hc", Mat)]
 
@@ -124,6 +134,9 @@ _SIGS = {
     # this library's own hooks (no reference counterpart)
     "zc_rank": ([P, U32, f32p, SZ, u32p, u32p, f32p], I32),
     "zc_bench_shift_map": ([P, f64p, U32, U32, U32, f64p], I32),
+    "zc_sweep": ([P, f64p, P, P, SZ, f32p, SZ], I32),
+    "zc_sweep_map": ([P, f32p, SZ, u32p], I32),
+    "zc_sweep_tune": ([U32], None),
 }
 
 LIB_NAME = "libzcsmi.so"

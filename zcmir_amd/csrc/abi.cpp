@@ -45,6 +45,44 @@ int32_t unsupported(Engine* e, const char* what) {
   return -1;
 }
 
+struct SweepInfo {
+  uint32_t nc, n, grid, n_peaks;
+  double cell, D, ox, oy;
+};
+static_assert(sizeof(SweepInfo) == 48, "SweepInfo layout");
+static_assert(sizeof(zc::SweepPeak) == 128, "SweepPeak layout");
+
+// The sweep options of the current settings (search.zig:663-711 runSweep).
+zc::SweepOpts sweep_opts(Engine* e, zc::SweepGrid& gv) {
+  const auto& s = e->set;
+  auto clampf = [](double x, double lo, double hi) { return std::min(std::max(x, lo), hi); };
+  auto clampu = [](uint32_t x, uint32_t lo, uint32_t hi) { return std::min(std::max(x, lo), hi); };
+  if (s.e("metric") != "smi") throw std::runtime_error("Unsupported: the sweep of this library scores metric smi");
+  const uint32_t g = s.u("sw_grid");
+  const double rot = clampf(s.f("sw_rot"), 0, 180);
+  const bool range = s.b("sw_rot_range");
+  const double lo = range ? clampf(s.f("sw_rot_lo"), -360, 360) : -rot;
+  double hi = range ? clampf(s.f("sw_rot_hi"), -360, 360) : rot;
+  if (hi < lo) hi += 360;
+  hi = std::min(hi, lo + 360);
+  gv = zc::sweep_grid(lo, hi, clampu(s.u("sw_nth"), 1, 720), clampf(s.f("sw_smin"), 0.05, 1),
+                      clampf(s.f("sw_smax"), 1, 20), clampu(s.u("sw_ns"), 1, 129),
+                      s.b("sw_shear") ? clampf(s.f("sw_shear_max"), 0, 1) : 0, clampu(s.u("sw_nshear"), 1, 15),
+                      s.b("sw_aniso") ? clampf(s.f("sw_aniso_max"), 0, 1) : 0, clampu(s.u("sw_naniso"), 1, 15));
+  zc::SweepOpts o{};
+  const double lw = e->sides[0].w, lh = e->sides[0].h, rw = e->sides[1].w, rh = e->sides[1].h;
+  o.H0 = s.b("sw_rel") ? e->H : Mat3{1, 0, (rw - lw) / 2, 0, 1, (rh - lh) / 2, 0, 0, 1};
+  o.gv = &gv;
+  o.grid = (g == 32 || g == 64 || g == 128 || g == 256) ? g : 128;
+  o.full = s.b("sw_full");
+  o.mode = s.b("exact") ? 1 : 0;
+  o.n_peak = 4;
+  const int tb = zc::SweepTune::batch;
+  o.batch = tb > 0 ? (uint32_t)tb : 64;
+  o.min_overlap = clampf(s.f("min_overlap"), 0, 0.5);
+  return o;
+}
+
 }  // namespace
 
 extern "C" {
@@ -311,6 +349,39 @@ int32_t zc_bench_shift_map(void* p, const double* H, uint32_t flags, uint32_t ca
     *ms_per_map = iters ? ms / iters : 0.0;
   });
 }
+
+// ── the sweep on its own (search.zig:300-494): raw peaks of every candidate, no NMS or climb ──
+// H0: the prior pose (null: the settings' rule, centred or the current pose with sw_rel).
+// Writes up to `cap` raw peaks and each candidate's best score; returns the peak count.
+int32_t zc_sweep(void* p, const double* H0, SweepInfo* info, zc::SweepPeak* peaks, size_t cap, float* best,
+                 size_t nbest) {
+  Engine* e = static_cast<Engine*>(p);
+  int32_t count = 0;
+  const int32_t rc = guard(e, "sweep", [&] {
+    e->need_images();
+    zc::SweepGrid gv;
+    zc::SweepOpts o = sweep_opts(e, gv);
+    if (H0) o.H0 = mat(H0);
+    const zc::SweepOut r = e->sweep(o);
+    if (info) *info = {r.nc, r.n, r.grid, (uint32_t)r.peaks.size(), r.cell, r.D, r.ox, r.oy};
+    if (peaks) {
+      const size_t n = std::min(cap, r.peaks.size());
+      std::memcpy(peaks, r.peaks.data(), n * sizeof(zc::SweepPeak));
+    }
+    if (best) std::memcpy(best, r.best.data(), std::min(nbest, r.best.size()) * sizeof(float));
+    count = (int32_t)r.peaks.size();
+  });
+  return rc < 0 ? rc : count;
+}
+
+// The score map of the last sweep batch's first candidate (nx × ny into dims).
+int32_t zc_sweep_map(void* p, float* map, size_t len, uint32_t* dims) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "sweep_map", [&] { e->read_sweep_map(map, len, dims); });
+}
+
+// Candidates per sweep batch (0: 64).
+void zc_sweep_tune(uint32_t batch) { zc::SweepTune::batch = (int)batch; }
 
 // ── reference entry points outside the SMI hot path: present so a ctypes binding of the
 //    reference's full symbol list loads, each failing with "Unsupported" ──

@@ -14,9 +14,6 @@
 
 namespace zc {
 
-static constexpr double kRidge = 1e-4;   // smi.zig:18
-static constexpr int kMinN = 12;          // smi.zig:17
-static constexpr int kMaxFft = 4096;      // longest FFT line the kernels implement
 
 // ── profiling ─────────────────────────────────────────────────────────────────────────────
 hipEvent_t Prof::get() {
@@ -69,12 +66,6 @@ Prof::~Prof() {
   }
 }
 
-#define LAUNCH(name, call) \
-  do {                     \
-    pre();                 \
-    call;                  \
-    post(name);            \
-  } while (0)
 
 // ── engine ────────────────────────────────────────────────────────────────────────────────
 Engine::Engine(int device) : dev(device) {
@@ -98,9 +89,10 @@ Engine::~Engine() {
     s.feat.release();
     s.mag.release();
     s.stack.release();
+    s.il.release();
   }
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
-                  &tmp_u32, &tmp_i64})
+                  &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks})
     b->release();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
@@ -142,6 +134,7 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
   sd.w = w;
   sd.h = h;
   sd.has = false;
+  sd.il_ok = false;
   sd.preprocess = preprocess;
   sd.gray.ensure(n * 4);
   sd.rank.ensure(n * 4);

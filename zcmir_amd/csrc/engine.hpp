@@ -14,9 +14,18 @@
 
 #include "geometry.hpp"
 #include "kernels.hpp"
+#include "search.hpp"
 #include "settings.hpp"
 
 namespace zc {
+
+// a profiled launch on the engine's stream
+#define LAUNCH(name, call) \
+  do {                     \
+    pre();                 \
+    call;                  \
+    post(name);            \
+  } while (0)
 
 #define ZC_HIP(expr)                                                                              \
   do {                                                                                            \
@@ -109,11 +118,17 @@ class Prof {
   hipEvent_t get();
 };
 
+inline constexpr double kRidge = 1e-4;   // smi.zig:18
+inline constexpr int kMinN = 12;          // smi.zig:17
+inline constexpr int kMaxFft = 4096;      // longest FFT line the kernels implement
+
 struct Side {
   int w = 0, h = 0;
   bool has = false;
   bool preprocess = false;
   DBuf gray, rank, feat, mag, stack;
+  DBuf il;   // features interleaved per pixel (float4) for the sweep's gathers
+  bool il_ok = false;
   float mean = 0, spread = 1;
   uint32_t hist[257] = {};
   uint32_t cdf[258] = {};
@@ -125,6 +140,29 @@ struct Side {
 // FFT tuning (zc_fft_tune): lines per workgroup for rows / columns, 0 = the default.
 struct FftTune {
   static inline std::atomic<int> rows{0}, cols{0};
+};
+
+// Sweep tuning: candidates per batch (0: 64, the reference's default).
+struct SweepTune {
+  static inline std::atomic<int> batch{0};
+};
+
+struct SweepOpts {
+  Mat3 H0;
+  const SweepGrid* gv;
+  uint32_t grid = 128;   // canvas cells per side
+  bool full = false;     // one cell per fixed pixel (grid = ceil(D))
+  uint32_t mode = 1;     // 0 global, 1 exact SMI
+  uint32_t n_peak = 4;
+  uint32_t batch = 64;
+  double min_overlap = 0.05;
+};
+
+struct SweepOut {
+  std::vector<SweepPeak> peaks;
+  std::vector<float> best;   // each candidate's map maximum (θ-major, then σ, shear, stretch)
+  uint32_t n = 0, nc = 0, grid = 0;
+  double cell = 0, D = 0, ox = 0, oy = 0;
 };
 
 struct MapOut {
@@ -167,6 +205,11 @@ class Engine {
   static double score_from_moments(const double* st, bool exact, double ridge);
 
   void fft2(float* data, int n, int planes, bool inverse);
+  // every candidate's shift map reduced to its top local maxima (search.zig:300-494)
+  SweepOut sweep(const SweepOpts& o);
+  // the map of the last sweep batch's first candidate (tests)
+  void read_sweep_map(float* out, size_t len, uint32_t dims[2]);
+  uint32_t sweep_last_nx = 0, sweep_last_ny = 0;
 
   void sync();
   // launch bookkeeping
@@ -178,9 +221,11 @@ class Engine {
   }
 
   DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
+  DBuf sw_poses, sw_pose_fixed, sw_zero, sw_ns, sw_part, sw_peaks;
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
   void ensure_acf(int s);
+  void ensure_il(int s);
 
  private:
   void prepare(int s);

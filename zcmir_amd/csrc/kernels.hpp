@@ -46,10 +46,12 @@ bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int
 // lines: columns per workgroup (0: the default for the length)
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
                    hipStream_t st, int lines = 0);
+// batch: spec_a and work hold `batch` consecutive entries (n_cplx / n_out planes each); spec_b
+// is shared
 void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* work, int nx, int ny,
-                 hipStream_t st);
+                 hipStream_t st, int batch = 1);
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
-                            hipStream_t st);
+                            hipStream_t st, int batch = 1);
 struct CombineArgs {
   int exact, calib, nf;
   float min_n;
@@ -67,5 +69,29 @@ bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw
 int moments_blocks(int w, int h);
 void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_b, int wb, int hb, const PoseArgs& p,
                          double* part, hipStream_t st);
+
+// ── rotation × scale sweep ──
+// A candidate: dest → source homography (row-major, 1-based pixels), taps per cell side and
+// the window's first canvas cell (sweep.wgsl:106-110).
+struct SweepPose {
+  float h[9];
+  int taps, gx0, gy0;
+};
+// One pack launch: the canvas (origin ox, oy in 1-based dest pixels, `cell` pixels per cell,
+// grid × grid cells), the source image (sw × sh) and the window (win_x × win_y cells) in an
+// nx × ny plane.
+struct SweepPack {
+  float ox, oy, cell;
+  int grid, sw, sh, win_x, win_y, ny;
+};
+void launch_interleave(const float* feat, int64_t n, float4* il, hipStream_t st);
+bool launch_pack_sweep(int plan, const float4* il, const SweepPose* poses, int batch, const SweepPack& u, int nx,
+                       float2* spec, const float2* tw_x, hipStream_t st);
+void launch_combine_sweep(int plan, const float* corr, float* ns, int64_t nn, int batch, const CombineArgs& a,
+                          const int* zero_lag, hipStream_t st);
+int peaks_chunks(int64_t nn);
+// part: batch · peaks_chunks(nx·ny) · 256 entries; peaks: batch · n_peak (value, lag bits)
+void launch_peaks(const float* ns, int nx, int ny, int batch, int n_peak, float2* part, float2* peaks,
+                  hipStream_t st);
 
 }  // namespace zc

@@ -120,6 +120,8 @@ static const SettingDef kSettings[] = {
     {"sw_k", Kind::U32, "32", nullptr},
     {"sw_fin", Kind::U32, "4", nullptr},
     {"sw_rel", Kind::Bool, "false", nullptr},
+    // not a reference option: the sweep at one canvas cell per fixed pixel (sw_grid ignored)
+    {"sw_full", Kind::Bool, "false", nullptr},
     {"hho_auto", Kind::Bool, "true", nullptr},
     {"hho_n", Kind::U32, "100", nullptr},
     {"hho_t", Kind::U32, "20", nullptr},
