@@ -97,7 +97,8 @@ void zc_swap(void* e);
 int32_t zc_moments(void* e, const double H[9], double out[45]);
 /* exports.zig:417: [mean, forward, inverse] (inverse NaN when not symmetric) */
 int32_t zc_score(void* e, const double H[9], double out[3]);
-/* exports.zig:424 (returns -1 "Unsupported" in this build) */
+/* exports.zig:424: score, overlap and tangent gradient at H (hess: also the Gauss-Newton
+ * matrix, nk x nk row-major), symmetric and exact per the settings; fwd, inv left 0 */
 int32_t zc_gradient(void* e, const double H[9], uint32_t hess, zc_grad_result* out);
 /* exports.zig:436: flags 1 exact | 2 calibrated, cap the FFT length cap (0: 1024), no
  * overlap floor; map (nullable) receives n_y x n floats */
@@ -117,9 +118,12 @@ int32_t zc_fft2(void* e, float* data, uint32_t n, uint32_t planes, uint32_t inve
 void zc_set_pose(void* e, const double H[9]);
 /* exports.zig:392 */
 void zc_get_pose(void* e, double H[9]);
-/* exports.zig:344 (returns -1 "Unsupported" in this build) */
+/* exports.zig:344: the sweep search (search_mode sweep, sw_* settings): sweep, NMS, a
+ * 4-iteration Gauss-Newton screen, 25 iterations on the finalists; the best pose becomes the
+ * engine's pose when it beats the current score */
 int32_t zc_search(void* e, zc_search_result* out);
-/* exports.zig:405 (returns -1 "Unsupported" in this build) */
+/* exports.zig:405: Gauss-Newton climb from the engine's pose (g_iters, g_a0, g_decay, g_steps;
+ * no shift-map hop or spline in this library); the result becomes the engine's pose */
 int32_t zc_climb(void* e, zc_climb_result* out);
 /* exports.zig:221 */
 void zc_cancel(void* e);

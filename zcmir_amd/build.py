@@ -17,9 +17,9 @@ OBJ = HERE / "_build"
 LIB = HERE / "libzcsmi.so"
 ARCH = os.environ.get("ZCSMI_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "engine.cpp", "sweep.cpp", "abi.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "abi.cpp"]
 HEADERS = ["fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
-           "search.hpp"]
+           "search.hpp", "lie.hpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "engine.hpp"
+#include "lie.hpp"
 
 using zc::Engine;
 using zc::Mat3;
@@ -374,6 +375,69 @@ int32_t zc_sweep(void* p, const double* H0, SweepInfo* info, zc::SweepPeak* peak
   return rc < 0 ? rc : count;
 }
 
+// exports.zig:424-427: score and tangent gradient at H (hess: also the Gauss–Newton matrix),
+// symmetric per the settings (engine.zig:786-797; fwd and inv are left 0 as there).
+int32_t zc_gradient(void* p, const double* H, uint32_t hess, zc::GradResult* out) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "gradient", [&] {
+    e->need_images();
+    const bool homog = e->set.e("group") == "homography";
+    const auto g = e->pair_grads(e->own_pair(1), {mat(H)}, e->set.b("exact"), homog, e->set.b("symmetric"),
+                                 hess != 0)[0];
+    std::memset(out, 0, sizeof *out);
+    const int nk = zc::n_keys(homog);
+    out->score = g.score;
+    out->n = g.n;
+    for (int k = 0; k < 8; ++k) out->grad[k] = g.grad[k];
+    if (hess) {
+      for (int k = 0; k < nk * nk; ++k) out->hess[k] = g.hess[k];
+      out->has_hess = 1;
+    }
+    out->nk = (uint32_t)nk;
+  });
+}
+
+// exports.zig:405-408: climb from the engine's pose; the result is the engine's new pose. This
+// library's climb is the batched Gauss–Newton refinement (g_iters iterations, the g_a0 / g_decay
+// / g_steps ascent ladder), without the reference's shift-map hops or spline.
+int32_t zc_climb(void* p, zc::ClimbResult* out) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "climb", [&] {
+    e->need_images();
+    e->cancel = false;
+    const auto& s = e->set;
+    const bool homog = s.e("group") == "homography";
+    const auto amps = zc::geo_amps(s.f("g_a0"), s.f("g_decay"), s.u("g_steps") > 0 ? s.u("g_steps") : 7);
+    uint32_t it = 0;
+    const auto r = e->gn_batch(e->own_pair(1), {e->H}, std::max<uint32_t>(1, s.u("g_iters")), homog, s.b("exact"),
+                               s.b("symmetric"), amps, &it)[0];
+    e->H = r.H;
+    for (int k = 0; k < 9; ++k) out->H[k] = r.H[k];
+    out->score = r.score;
+    out->iterations = it;
+    out->moved = r.moved ? 1 : 0;
+  });
+}
+
+// exports.zig:376-380 / search.zig:646-822: the sweep search with the current settings; the best
+// pose replaces the engine's when it scores higher.
+int32_t zc_search(void* p, zc::SearchResult* out) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "search", [&] {
+    e->need_images();
+    e->cancel = false;
+    const auto& s = e->set;
+    if (s.e("search_mode") != "sweep") throw std::runtime_error("Unsupported: search_mode cloud");
+    zc::SweepGrid gv;
+    const zc::SweepOpts o = sweep_opts(e, gv);
+    const uint32_t k = s.u("sw_k");
+    const uint32_t keep = (k == 8 || k == 16 || k == 32) ? k : 32;
+    const uint32_t n_fin = std::min<uint32_t>(std::max<uint32_t>(s.u("sw_fin"), 1), 16);
+    const auto amps = zc::geo_amps(s.f("g_a0"), s.f("g_decay"), s.u("g_steps") > 0 ? s.u("g_steps") : 7);
+    *out = e->search(o, keep, n_fin, s.e("group") == "homography", s.b("exact"), s.b("symmetric"), amps);
+  });
+}
+
 // The score map of the last sweep batch's first candidate (nx × ny into dims).
 int32_t zc_sweep_map(void* p, float* map, size_t len, uint32_t* dims) {
   Engine* e = static_cast<Engine*>(p);
@@ -407,9 +471,6 @@ ZC_UNSUPPORTED(zc_overlay, void* e, const double*, uint32_t, void*, float*, floa
 ZC_UNSUPPORTED(zc_tile_heat, void* e, const double*, uint32_t, uint32_t, double*, float*, float*, size_t)
 ZC_UNSUPPORTED(zc_displacement, void* e, float*, size_t)
 ZC_UNSUPPORTED(zc_rs_map, void* e, const double*, const double*, uint32_t, void*, float*, float*, float*, size_t)
-ZC_UNSUPPORTED(zc_search, void* e, void*)
-ZC_UNSUPPORTED(zc_climb, void* e, void*)
-ZC_UNSUPPORTED(zc_gradient, void* e, const double*, uint32_t, void*)
 
 int64_t zc_export_warp(void* e, const char*, size_t, void*, size_t) { return unsupported((Engine*)e, "zc_export_warp"); }
 size_t zc_get_ffd(void* e, float*, size_t) {

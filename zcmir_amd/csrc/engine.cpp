@@ -92,7 +92,8 @@ Engine::~Engine() {
     s.il.release();
   }
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
-                  &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks})
+                  &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
+                  &gn_sums})
     b->release();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);

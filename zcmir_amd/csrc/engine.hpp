@@ -165,11 +165,44 @@ struct SweepOut {
   double cell = 0, D = 0, ox = 0, oy = 0;
 };
 
+// Score and gradient coefficients of one direction's 45 sums (moments.zig:24-28 Stats).
+struct Coefs {
+  double score, n;
+  double W[16], Mh[16], Q[16], ma[4], mb[4], g[45];
+};
+Coefs score_coefs(const double* st, bool exact, double ridge);
+
+// A pose's score, overlap, tangent gradient and Gauss–Newton matrix (nk × nk row-major)
+// (pose.zig:76-82 Grad).
+struct GradOut {
+  double score, n;
+  double grad[8];
+  double hess[64];
+  bool has_hess;
+};
+struct GnOut {
+  Mat3 H;
+  double score;
+  bool moved;
+};
+
 struct MapOut {
   Frame fr;
   GridPlan g;
   bool exact, calib;
   float res[3];
+};
+
+// One direction of a pose: side a's features sampled onto side b's pixels at H (a → b).
+struct DirJob {
+  const Side* a;
+  const Side* b;
+  Mat3 H;
+};
+// An image pair: moving side a, fixed side b.
+struct PairSides {
+  const Side* a;
+  const Side* b;
 };
 
 class Engine {
@@ -207,6 +240,20 @@ class Engine {
   void fft2(float* data, int n, int planes, bool inverse);
   // every candidate's shift map reduced to its top local maxima (search.zig:300-494)
   SweepOut sweep(const SweepOpts& o);
+  // batched pose evaluation (gn.cpp)
+  void moments_jobs(const std::vector<DirJob>& d, std::vector<std::array<double, 45>>& out);
+  void tangent_jobs(const std::vector<DirJob>& d, const std::vector<Coefs>& co, bool homography, bool hess,
+                    std::vector<std::array<double, 44>>& out);
+  std::vector<double> pair_scores(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, bool exact,
+                                  bool symmetric);
+  std::vector<GradOut> pair_grads(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, bool exact,
+                                  bool homography, bool symmetric, bool hess);
+  std::vector<GnOut> gn_batch(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, uint32_t iters,
+                              bool homography, bool exact, bool symmetric, const std::vector<double>& amps,
+                              uint32_t* iterations = nullptr);
+  std::vector<PairSides> own_pair(size_t n) const;
+  SearchResult search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, bool homography, bool exact,
+                      bool symmetric, const std::vector<double>& amps);
   // the map of the last sweep batch's first candidate (tests)
   void read_sweep_map(float* out, size_t len, uint32_t dims[2]);
   uint32_t sweep_last_nx = 0, sweep_last_ny = 0;
@@ -222,6 +269,7 @@ class Engine {
 
   DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
   DBuf sw_poses, sw_pose_fixed, sw_zero, sw_ns, sw_part, sw_peaks;
+  DBuf gn_jobs, gn_part, gn_sums;
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
   void ensure_acf(int s);

@@ -70,6 +70,32 @@ int moments_blocks(int w, int h);
 void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_b, int wb, int hb, const PoseArgs& p,
                          double* part, hipStream_t st);
 
+// ── batched pose sums and gradients (one pose per job) ──
+// moving features fa (wa × ha, 4 planes) sampled at Hi · X for every pixel X of fb (wb × hb)
+struct PoseJob {
+  const float* fa;
+  const float* fb;
+  int wa, ha, wb, hb;
+  double Hi[9];
+};
+// the tangent pass of one pose: Hi, M_k = −Hi · G_k (pixel generators of fb's image, zero past
+// the group's), the score's coefficients (W, M̂, Q row-major, the means, g = ∂S/∂sums)
+struct TangentJob {
+  const float* fa;
+  const float* fb;
+  int wa, ha, wb, hb, hess, _pad;
+  double Hi[9];
+  float Mk[72];
+  float W[16], Mh[16], Q[16], ma[4], mb[4], g[45];
+};
+int jobs_blocks(int64_t max_pixels);
+// 45 sums per job into sums (part: njobs · blocks · 45 doubles)
+void launch_pose_moments_jobs(const PoseJob* jobs, int njobs, int blocks, double* part, double* sums,
+                              hipStream_t st);
+// 44 sums per job (8 gradient + 36 upper-triangle matrix) into sums (part: njobs · blocks · 44)
+void launch_tangent_jobs(const TangentJob* jobs, int njobs, int blocks, double* part, double* sums,
+                         hipStream_t st);
+
 // ── rotation × scale sweep ──
 // A candidate: dest → source homography (row-major, 1-based pixels), taps per cell side and
 // the window's first canvas cell (sweep.wgsl:106-110).
