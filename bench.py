@@ -57,6 +57,84 @@ def calibrate_copy(mb):
     print(json.dumps({"copy_bytes": n * 4, "reads": n * 4, "writes": n * 4}))
 
 
+def _kernel_table(prof, pbytes, per):
+    """Per-kernel launches, times and achieved algorithmic GB/s (bytes accounted by the engine)."""
+    out = {}
+    for name, (cnt, tot, avg) in prof.items():
+        b = pbytes.get(name)
+        out[name] = {"launches_per_unit": cnt / per, "avg_ms": avg, "ms_per_unit": tot / per,
+                     "alg_bytes_per_unit": (b / per) if b else None,
+                     "gbs": (b / (tot * 1e-3) / 1e9) if b else None}
+    return out
+
+
+def _roofline(kernels, traffic=None):
+    dom = max((k for k in kernels if kernels[k].get("gbs")), key=lambda k: kernels[k]["ms_per_unit"])
+    d = kernels[dom]
+    return {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "alg_bytes_per_launch": d["alg_bytes_per_unit"] / d["launches_per_unit"],
+            "avg_launch_ms": round(d["avg_ms"], 5)}
+
+
+SWEEP_GRID = dict(sw_nth=72, sw_smin=0.8, sw_smax=1.25, sw_ns=9)   # 72 θ (5° steps) × 9 σ, 0.8 – 1.25
+
+
+def bench_sweep(a, np, torch, zcmir_amd, synth):
+    """Sweep poses/s: a 1024² pair whose moving image is the fixed one rotated 37°, scaled 1.12
+    (bilinear) and remapped by x^0.5; 648 candidate poses, each a full SMI translation map reduced
+    to its top local maxima. Two canvases: 128² cells (the reference's default grid, footprint-
+    sized FFTs) and one cell per fixed pixel (full resolution)."""
+    mv, fx, _ = synth.rotscale_pair((1024, 1024), theta_deg=37.0, scale=1.12, seed=2)
+    out = {}
+    for label, kw, steps in (("grid128", dict(sw_grid=128), a.sweep_steps),
+                             ("full_resolution", dict(sw_full=True), a.sweep_full_steps)):
+        if steps <= 0:
+            continue
+        reg = zcmir_amd.Registrar(half=False, symmetric=True, **SWEEP_GRID, **kw)
+        reg.set_images(mv, fx, preprocess=False)
+        info, peaks, best = reg.sweep()
+        k = int(np.argmax(best))
+        th_deg, sg = -180 + 5 * (k // 9), 1.25 ** ((k % 9 - 4) / 4)
+        if th_deg != -35 or abs(sg - 1.25 ** -0.5) > 1e-9:
+            raise SystemExit(f"sweep workload check failed ({label}): best candidate θ {th_deg}, σ {sg:.4f}")
+        torch.cuda.synchronize()
+        reg.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reg.sweep()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernels = _kernel_table(reg.profile_report(), reg.profile_bytes(), steps)
+        reg.profile(False)
+        out[label] = {
+            "value": round(info["nc"] * steps / wall, 2), "unit": "poses/s", "steps": steps,
+            "ms_per_sweep": round(1e3 * wall / steps, 3), "candidates": info["nc"], "grid": info["grid"],
+            "cell_px": round(info["cell"], 4), "largest_fft": info["n"],
+            "roofline": _roofline(kernels),
+            "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                        for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_unit"])},
+        }
+        reg.close()
+    if not a.no_cpu_baseline and "grid128" in out:
+        from oracle import smi_ref as O
+
+        pm, pf = O.prepare(mv), O.prepare(fx)
+        n_th, t0, done = 0, time.perf_counter(), 0
+        while time.perf_counter() - t0 < a.cpu_budget / 2 and n_th < 72:
+            th = -180 + 5 * n_th
+            gv = O.sweep_grid(th, th, 1, 0.8, 1.25, 9)
+            O.sweep(pm, pf, np.eye(3), gv, grid=128)
+            done += 9
+            n_th += 1
+        el = time.perf_counter() - t0
+        out["grid128"]["cpu_baseline"] = {
+            "value": round(done / el, 3), "unit": "poses/s", "cores": 1, "kind": "port",
+            "sample": f"{done} of the 648 candidates ({n_th} θ × 9 σ) on the 128² canvas in {el:.1f} s, "
+                      "float64 NumPy restatement of the sweep (one thread)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,11 +145,13 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a copy kernel of this many MiB and exit")
     ap.add_argument("--fft-cols", type=int, default=0, help="columns per workgroup of the column FFT (0: default)")
+    ap.add_argument("--sweep-steps", type=int, default=10, help="timed sweeps on the 128² canvas (0: skip)")
+    ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
     a = ap.parse_args()
 
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path shards by "
-              "candidate / slice, never within one map; see DESIGN.md)", file=sys.stderr)
+              "candidate / slice, never within one map)", file=sys.stderr)
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
@@ -154,7 +234,7 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"config 1: 1024x1024 pair, shift (113,-57), 8-bit intensities, {a.corr} "
+        "config": {"workload": f"1024x1024 pair, shift (113,-57), 8-bit intensities, {a.corr} "
                                "correlations, full-resolution 2048x2048 FFT, exact non-calibrated SMI",
                    "image": list(size), "fft": [sm.n, sm.ny], "correlations": a.corr,
                    "per_lag_whitening": "f64", "parallelism": "single GPU"},
@@ -169,6 +249,11 @@ def main():
         "device": reg.adapter,
     }
     reg.close()
+    if a.sweep_steps > 0 or a.sweep_full_steps > 0:
+        line["sweep"] = {"metric": "sweep poses/s", "workload": "1024x1024 pair, moving = fixed rotated 37 deg, "
+                         "scaled 1.12 (bilinear), remapped x^0.5; 72 rotations x 9 scales (0.8-1.25) = 648 "
+                         "candidates, exact SMI, top-4 local maxima per candidate map",
+                         **bench_sweep(a, np, torch, zcmir_amd, synth)}
     print(json.dumps(line))
 
 

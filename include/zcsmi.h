@@ -175,6 +175,18 @@ int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
 /* candidates per sweep batch (0: 64); process-wide */
 void zc_sweep_tune(uint32_t batch);
 
+/* stacks of uneven slice pairs (the role of python/zcmir/api.py:522-580 register_stack, batched):
+ * slice i = moving[i] (moving_wh[2i] x moving_wh[2i+1] f32 gray, taken as the work image) onto
+ * fixed[i]; features stay resident until the next zc_stack_set */
+int32_t zc_stack_set(void* e, uint32_t n, const float* const* moving, const uint32_t* moving_wh,
+                     const float* const* fixed, const uint32_t* fixed_wh);
+/* H0: n x 9 start poses (NULL: centred); init_map: 1 correct each by its SMI shift map's peak
+ * first, 2 by the best peak of a rotation x scale sweep about it (+-12 deg, 0.88-1.12), 0 neither;
+ * then `iters` Gauss-Newton iterations on all slices at once. Writes H_out (n x 9), scores (n) and
+ * H_init (n x 9, nullable: the poses after the map); returns the iterations run */
+int32_t zc_stack_register(void* e, const double* H0, uint32_t iters, uint32_t init_map, double* H_out,
+                          double* scores, double* H_init);
+
 /* ── reference entry points outside the SMI hot path (exports.zig:95-138, 232, 245, 284-376,
  *    382, 397, 430, 458-489): exported so the reference's full ctypes symbol list binds; each
  *    fails with zc_error "Unsupported" (or is a no-op tuning hook) ── */

@@ -913,10 +913,10 @@ def nms4(peaks, n_theta: int, keep: int, wrap: bool):
     for p in sorted(peaks, key=lambda q: -q["score"]):
         ok = True
         for q in kept:
-            a = abs(p["it"] - q["it"])
-            dt = min(a, n_theta - a) if wrap else a
-            if (dt <= 2 and abs(p["is_"] - q["is_"]) <= 2 and abs(p["ik"] - q["ik"]) <= 1 and abs(p["ia"] - q["ia"]) <= 1
-                    and math.hypot(p["sx"] - q["sx"], p["sy"] - q["sy"]) <= 4):
+            d = {k: abs(int(p[k]) - int(q[k])) for k in ("it", "is_", "ik", "ia", "sx", "sy")}
+            dt = min(d["it"], n_theta - d["it"]) if wrap else d["it"]
+            if (dt <= 2 and d["is_"] <= 2 and d["ik"] <= 1 and d["ia"] <= 1
+                    and math.hypot(d["sx"], d["sy"]) <= 4):
                 ok = False
                 break
         if ok:

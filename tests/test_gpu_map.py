@@ -77,7 +77,7 @@ def _map_case(reg_factory, mv, fx, H, exact=True, calibrated=False, mode="laplac
 @pytest.mark.parametrize("exact,calibrated,mode", [(True, False, "laplacian"), (True, False, "three"),
                                                    (False, False, "laplacian"), (True, True, "laplacian"),
                                                    (True, False, "cubic")])
-def test_shift_map_256_config0(reg_factory, exact, calibrated, mode):
+def test_shift_map_256_shift_17_m9(reg_factory, exact, calibrated, mode):
     mv, fx, Ht = _pair((256, 256), (17, -9))
     sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), exact, calibrated, mode)
     assert (sm.n, sm.ny, sm.cw, sm.ch) == (512, 512, 256, 256)
@@ -111,7 +111,7 @@ def test_shift_map_warped_resampled(reg_factory):
 
 
 @pytest.mark.parametrize("mode", ["laplacian", "three"])
-def test_shift_map_1024_config1(reg_factory, mode):
+def test_shift_map_1024_full_resolution(reg_factory, mode):
     """1024² pair, 8-bit intensities, shift (113, -57), full-resolution 2048² FFT."""
     mv, fx, _ = _pair((1024, 1024), (113, -57), seed=11, quantize=True)
     sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), mode=mode, cap=2048)

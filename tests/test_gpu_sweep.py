@@ -53,7 +53,7 @@ def test_sweep_candidate_map(reg_factory, theta, sigma, grid, exact, mode):
     assert info["nc"] == 1 and info["grid"] == grid and abs(info["cell"] - r["cell"]) < 1e-12
     err, scale = _compare_map(gm, r["maps"][0])
     assert err <= TOL, (err, scale)
-    assert peaks and r["peaks"]
+    assert len(peaks) > 0 and len(r["peaks"]) > 0
     assert (peaks[0]["sx"], peaks[0]["sy"]) == (r["peaks"][0]["sx"], r["peaks"][0]["sy"])
     assert abs(peaks[0]["score"] - r["peaks"][0]["score"]) <= TOL * r["peaks"][0]["score"]
     assert np.allclose(peaks[0]["H"], r["peaks"][0]["H"], rtol=1e-12, atol=1e-9)

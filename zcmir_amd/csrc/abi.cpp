@@ -438,6 +438,52 @@ int32_t zc_search(void* p, zc::SearchResult* out) {
   });
 }
 
+// ── stacks: many slice pairs registered together (the role of python/zcmir/api.py:522-580) ──
+// Slice i: moving[i] (moving_wh[2i] × moving_wh[2i+1] f32 gray, used as the work image) and
+// fixed[i]; their features stay resident until the next zc_stack_set.
+int32_t zc_stack_set(void* p, uint32_t n, const float* const* moving, const uint32_t* moving_wh,
+                     const float* const* fixed, const uint32_t* fixed_wh) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "stack_set", [&] { e->stack_set(n, moving, moving_wh, fixed, fixed_wh); });
+}
+
+// Register every slice of the stack: start poses H0 (n × 9, NULL: each moving image centred on
+// its fixed one), init_map: 1 first correct each start by its SMI shift map's peak (map_res,
+// min_overlap, exact), 2 by the best peak of a small rotation × scale sweep about it (±12°,
+// 0.88 – 1.12), 0 neither; then `iters` Gauss–Newton iterations on all slices at once (group,
+// symmetric, exact, g_a0 / g_decay / g_steps). Writes the poses (n × 9), their scores (n) and,
+// if not NULL, the poses after the map (n × 9); returns the Gauss–Newton iterations run.
+int32_t zc_stack_register(void* p, const double* H0, uint32_t iters, uint32_t init_map, double* H_out,
+                          double* scores, double* H_init) {
+  Engine* e = static_cast<Engine*>(p);
+  uint32_t it = 0;
+  const int32_t rc = guard(e, "stack_register", [&] {
+    const size_t n = e->stack.size() / 2;
+    std::vector<Mat3> h0(n);
+    for (size_t i = 0; i < n; ++i) {
+      if (H0) {
+        h0[i] = mat(H0 + 9 * i);
+      } else {
+        const double lw = e->stack[2 * i]->w, lh = e->stack[2 * i]->h;
+        const double rw = e->stack[2 * i + 1]->w, rh = e->stack[2 * i + 1]->h;
+        h0[i] = {1, 0, (rw - lw) / 2, 0, 1, (rh - lh) / 2, 0, 0, 1};
+      }
+    }
+    std::vector<Mat3> hinit;
+    e->cancel = false;
+    if (init_map > 2) throw std::runtime_error("BadArgument: init 0 (none), 1 (shift map) or 2 (sweep)");
+    const auto r = e->stack_register(h0, iters, (int)init_map, &hinit, &it);
+    for (size_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 9; ++k) {
+        H_out[9 * i + k] = r[i].H[k];
+        if (H_init) H_init[9 * i + k] = hinit[i][k];
+      }
+      if (scores) scores[i] = r[i].score;
+    }
+  });
+  return rc < 0 ? rc : (int32_t)it;
+}
+
 // The score map of the last sweep batch's first candidate (nx × ny into dims).
 int32_t zc_sweep_map(void* p, float* map, size_t len, uint32_t* dims) {
   Engine* e = static_cast<Engine*>(p);

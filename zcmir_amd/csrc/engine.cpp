@@ -52,8 +52,9 @@ std::string Prof::report() const {
   std::ostringstream o;
   for (const auto& kv : acc_) {
     char line[256];
-    std::snprintf(line, sizeof line, "%s %ld %.6f %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second,
-                  kv.second.second / std::max(1L, kv.second.first));
+    const auto b = bytes_.find(kv.first);
+    std::snprintf(line, sizeof line, "%s %ld %.6f %.6f %.0f\n", kv.first.c_str(), kv.second.first, kv.second.second,
+                  kv.second.second / std::max(1L, kv.second.first), b == bytes_.end() ? 0.0 : b->second);
     o << line;
   }
   return o.str();
@@ -83,14 +84,8 @@ Engine::Engine(int device) : dev(device) {
 Engine::~Engine() {
   (void)hipSetDevice(dev);
   if (st) (void)hipStreamSynchronize(st);
-  for (auto& s : sides) {
-    s.gray.release();
-    s.rank.release();
-    s.feat.release();
-    s.mag.release();
-    s.stack.release();
-    s.il.release();
-  }
+  for (auto& s : sides) release_side(s);
+  for (auto& s : stack) release_side(*s);
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
                   &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
                   &gn_sums})
@@ -98,6 +93,12 @@ Engine::~Engine() {
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
   if (st) (void)hipStreamDestroy(st);
+}
+
+void Engine::release_side(Side& s) {
+  for (DBuf* b : {&s.gray, &s.rank, &s.feat, &s.mag, &s.stack, &s.il}) b->release();
+  s.has = false;
+  s.il_ok = false;
 }
 
 void Engine::sync() {
@@ -143,14 +144,13 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
   sd.mag.ensure(n * 4);
   sd.stack.ensure(n * 20);
   ZC_HIP(hipMemcpyAsync(sd.gray.p, data, n * 4, hipMemcpyHostToDevice, st));
-  prepare(s);
+  prepare(sd, true);
   sd.has = true;
 }
 
 // Smi.prepare (smi.zig:348-400) for SMI features, with exact image statistics (see the oracle's
 // definition: order-independent integer sums, so the rank bins are reproducible bit for bit).
-void Engine::prepare(int s) {
-  Side& sd = sides[s];
+void Engine::prepare(Side& sd, bool acf) {
   const int64_t n = (int64_t)sd.w * sd.h;
   float* gray = sd.gray.as<float>();
   sd.half = set.b("half");
@@ -244,12 +244,11 @@ void Engine::prepare(int s) {
     }
   for (int i = 0; i < 4; ++i) wa.mu[i] = mu[i];
   LAUNCH("whiten", launch_whiten(sd.feat.as<float>(), sd.stack.as<float>(), n, wa, st));
-  ensure_acf(s);
+  if (acf) ensure_acf(sd);
 }
 
 // Normalized feature autocorrelation (smi.wgsl:936-975, 1574-1580).
-void Engine::ensure_acf(int s) {
-  Side& sd = sides[s];
+void Engine::ensure_acf(Side& sd) {
   const int nb = acf_blocks(sd.w, sd.h);
   parts.ensure((size_t)nb * 1156 * 8);
   sums.ensure(2048 * 8);
@@ -270,13 +269,13 @@ void Engine::ensure_acf(int s) {
 }
 
 // Bartlett correlation area of the pair, px² (smi.wgsl:1582-1595).
-double Engine::pair_area() const {
+double Engine::pair_area(const Side& A, const Side& B) const {
   double s = 0;
   for (int k = 0; k < 289; ++k) {
     double ra = 0, rb = 0;
     for (int i = 0; i < 4; ++i) {
-      ra += sides[0].acf[k * 4 + i];
-      rb += sides[1].acf[k * 4 + i];
+      ra += A.acf[k * 4 + i];
+      rb += B.acf[k * 4 + i];
     }
     s += ra * rb;
   }
@@ -286,8 +285,11 @@ double Engine::pair_area() const {
 // The dense shift map of pose H (smi.zig:540-572; engine.zig:795-819).
 MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, double min_overlap, bool wait) {
   need_images();
-  Side& A = sides[0];
-  Side& B = sides[1];
+  return shift_map_sides(sides[0], sides[1], Hp, exact, calib, cap, min_overlap, wait);
+}
+
+MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, bool calib, uint32_t cap,
+                               double min_overlap, bool wait) {
   if (A.mode != B.mode) throw std::runtime_error("FeatureModeMismatch: set both images with the same smi_features");
   const bool three = A.mode == "three";
   const int plan = (exact ? 1 : 0) + (three ? 2 : 0);
@@ -356,7 +358,7 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
   ca.min_n = (float)std::max<uint32_t>(kMinN, floor_n);
   ca.ridge = kRidge;
   const double gcell = (double)W / (double)cw;
-  ca.area = std::max(1.0, pair_area() / (gcell * gcell));
+  ca.area = std::max(1.0, pair_area(A, B) / (gcell * gcell));
   LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));

@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -103,7 +104,12 @@ class Prof {
   void end(const char* name, hipStream_t st);
   void collect();   // after a synchronization
   std::string report() const;
-  void reset() { acc_.clear(); }
+  void reset() {
+    acc_.clear();
+    bytes_.clear();
+  }
+  // algorithmic HBM bytes of a launch (minimal traffic of its role), summed per kernel name
+  void add_bytes(const char* name, double b) { bytes_[name] += b; }
   ~Prof();
 
  private:
@@ -115,6 +121,7 @@ class Prof {
   std::vector<Pending> pending_;
   hipEvent_t cur_ = nullptr;
   std::map<std::string, std::pair<long, double>> acc_;
+  std::map<std::string, double> bytes_;
   hipEvent_t get();
 };
 
@@ -224,11 +231,14 @@ class Engine {
   void set_image(int side, const float* data, int w, int h, bool preprocess);
   bool has_images() const { return sides[0].has && sides[1].has; }
   void need_images();
-  double pair_area() const;
+  double pair_area() const { return pair_area(sides[0], sides[1]); }
+  double pair_area(const Side& A, const Side& B) const;
 
   // map of H into the engine's map buffer; returns its plan and [peak, idx, map[0]]
   // wait=false: enqueue only (the result lands in res_host at the next synchronization)
   MapOut shift_map(const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap, bool wait = true);
+  MapOut shift_map_sides(Side& A, Side& B, const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap,
+                         bool wait = true);
   float* res_host = nullptr;   // pinned
   void fill_shift_result(const MapOut& m, ShiftResult* out) const;
   void read_map(float* out, size_t len, const MapOut& m);
@@ -240,6 +250,7 @@ class Engine {
   void fft2(float* data, int n, int planes, bool inverse);
   // every candidate's shift map reduced to its top local maxima (search.zig:300-494)
   SweepOut sweep(const SweepOpts& o);
+  SweepOut sweep_sides(Side& A, Side& B, const SweepOpts& o);
   // batched pose evaluation (gn.cpp)
   void moments_jobs(const std::vector<DirJob>& d, std::vector<std::array<double, 45>>& out);
   void tangent_jobs(const std::vector<DirJob>& d, const std::vector<Coefs>& co, bool homography, bool hess,
@@ -254,6 +265,15 @@ class Engine {
   std::vector<PairSides> own_pair(size_t n) const;
   SearchResult search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, bool homography, bool exact,
                       bool symmetric, const std::vector<double>& amps);
+  // stacks (stack.cpp): slice pairs with their own features, registered together
+  std::vector<std::unique_ptr<Side>> stack;   // moving, fixed per slice
+  void stack_set(uint32_t n, const float* const* mv, const uint32_t* mwh, const float* const* fx,
+                 const uint32_t* fwh);
+  // init: 0 the start poses, 1 each corrected by its shift map's peak, 2 by the best peak of a
+  // rotation × scale sweep about it (rotation ±12° in 4° steps × scale 0.88 – 1.12 in 5, 64² canvas)
+  std::vector<GnOut> stack_register(const std::vector<Mat3>& H0, uint32_t iters, int init, std::vector<Mat3>* Hinit,
+                                    uint32_t* iterations);
+  static void release_side(Side& s);
   // the map of the last sweep batch's first candidate (tests)
   void read_sweep_map(float* out, size_t len, uint32_t dims[2]);
   uint32_t sweep_last_nx = 0, sweep_last_ny = 0;
@@ -261,6 +281,9 @@ class Engine {
   void sync();
   // launch bookkeeping
   void pre() { if (prof.on) prof.begin(st); }
+  void bytes(const char* name, double b) {
+    if (prof.on) prof.add_bytes(name, b);
+  }
   void post(const char* name) {
     ++counts[0];
     if (prof.on) prof.end(name, st);
@@ -272,11 +295,11 @@ class Engine {
   DBuf gn_jobs, gn_part, gn_sums;
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
-  void ensure_acf(int s);
-  void ensure_il(int s);
+  void ensure_acf(Side& sd);
+  void ensure_il(Side& sd);
 
  private:
-  void prepare(int s);
+  void prepare(Side& sd, bool acf);
 };
 
 }  // namespace zc

@@ -1080,6 +1080,34 @@ __device__ __forceinline__ void sweep_cell(const float4* __restrict__ il, const 
   float a0 = 0.0f;
   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float xmax = (float)(u.sw - 1), ymax = (float)(u.sh - 1);
+  {
+    // A cell whose four corners map (with one sign of the homogeneous coordinate) beyond one
+    // edge of the source, by more than rounding, has no valid tap: its values are zero.
+    float cxs[4], cys[4];
+    bool same = true;
+    float zs = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float px = u.ox + (float)(cx + (c & 1)) * u.cell, py = u.oy + (float)(cy + (c >> 1)) * u.cell;
+      const float qz = pz.h[6] * px + pz.h[7] * py + pz.h[8];
+      if (c == 0) zs = qz;
+      same &= (qz > 1e-8f && zs > 1e-8f) || (qz < -1e-8f && zs < -1e-8f);
+      cxs[c] = (pz.h[0] * px + pz.h[1] * py + pz.h[2]) / qz - 1.0f;
+      cys[c] = (pz.h[3] * px + pz.h[4] * py + pz.h[5]) / qz - 1.0f;
+    }
+    if (same) {
+      const float m = 1e-3f;
+      const bool left = cxs[0] < -m && cxs[1] < -m && cxs[2] < -m && cxs[3] < -m;
+      const bool right = cxs[0] > xmax + m && cxs[1] > xmax + m && cxs[2] > xmax + m && cxs[3] > xmax + m;
+      const bool top = cys[0] < -m && cys[1] < -m && cys[2] < -m && cys[3] < -m;
+      const bool bottom = cys[0] > ymax + m && cys[1] > ymax + m && cys[2] > ymax + m && cys[3] > ymax + m;
+      if (left || right || top || bottom) {
+#pragma unroll
+        for (int p = 0; p < 5; ++p) s[p] = 0.0f;
+        return;
+      }
+    }
+  }
   for (int ty = 0; ty < k; ++ty) {
     const float py = u.oy + ((float)cy + ((float)ty + 0.5f) / fk) * u.cell;
     for (int tx = 0; tx < k; ++tx) {
@@ -1182,9 +1210,11 @@ __global__ __launch_bounds__(256) void k_combine_sweep(const float* __restrict__
 // the lags t ≡ c (mod 256) and keeps its largest 3×3 local maximum (periodic neighbours; ties:
 // the first lag); the n_peak best classes are the peaks. Split over chunks of lags: part holds
 // (value, lag) per (entry, chunk, class).
-__global__ __launch_bounds__(256) void k_peaks_part(const float* __restrict__ ns, int nx, int ny, int64_t chunk,
+// nx, ny: powers of two (lx = log2 nx), so the periodic neighbours are masks.
+__global__ __launch_bounds__(256) void k_peaks_part(const float* __restrict__ ns, int lx, int ny, int64_t chunk,
                                                     float2* part) {
   const int b = blockIdx.y, lid = threadIdx.x;
+  const int nx = 1 << lx, mx = nx - 1, my = ny - 1;
   const int64_t nn = (int64_t)nx * ny;
   const float* m = ns + (int64_t)b * nn;
   const int64_t t0 = (int64_t)blockIdx.x * chunk, t1 = min(nn, t0 + chunk);
@@ -1193,17 +1223,15 @@ __global__ __launch_bounds__(256) void k_peaks_part(const float* __restrict__ ns
   for (int64_t t = t0 + lid; t < t1; t += 256) {
     const float v = m[t];
     if (v <= bv) continue;
-    const int y = (int)(t / nx), x = (int)(t - (int64_t)y * nx);
+    const int y = (int)(t >> lx), x = (int)t & mx;
     bool ok = true;
-    for (int dy = -1; dy <= 1 && ok; ++dy) {
-      const int yy = (y + dy + ny) % ny;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int64_t row = (int64_t)((y + dy) & my) << lx;
+#pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         if (dx == 0 && dy == 0) continue;
-        const int xx = (x + dx + nx) % nx;
-        if (m[(int64_t)yy * nx + xx] > v) {
-          ok = false;
-          break;
-        }
+        ok &= !(m[row + ((x + dx) & mx)] > v);
       }
     }
     if (ok) {
@@ -1479,7 +1507,9 @@ void launch_peaks(const float* ns, int nx, int ny, int batch, int n_peak, float2
   const int64_t nn = (int64_t)nx * ny;
   const int chunks = peaks_chunks(nn);
   const int64_t chunk = ((nn + chunks - 1) / chunks + 255) / 256 * 256;
-  k_peaks_part<<<dim3(chunks, batch), 256, 0, st>>>(ns, nx, ny, chunk, part);
+  int lx = 0;
+  while ((1 << lx) < nx) ++lx;
+  k_peaks_part<<<dim3(chunks, batch), 256, 0, st>>>(ns, lx, ny, chunk, part);
   k_peaks_final<<<batch, 256, 0, st>>>(part, chunks, n_peak, peaks);
 }
 

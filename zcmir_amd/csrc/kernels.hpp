@@ -116,7 +116,8 @@ bool launch_pack_sweep(int plan, const float4* il, const SweepPose* poses, int b
 void launch_combine_sweep(int plan, const float* corr, float* ns, int64_t nn, int batch, const CombineArgs& a,
                           const int* zero_lag, hipStream_t st);
 int peaks_chunks(int64_t nn);
-// part: batch · peaks_chunks(nx·ny) · 256 entries; peaks: batch · n_peak (value, lag bits)
+// nx, ny powers of two; part: batch · peaks_chunks(nx·ny) · 256 entries; peaks: batch · n_peak
+// (value, lag bits)
 void launch_peaks(const float* ns, int nx, int ny, int batch, int n_peak, float2* part, float2* peaks,
                   hipStream_t st);
 

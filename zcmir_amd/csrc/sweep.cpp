@@ -17,8 +17,7 @@
 
 namespace zc {
 
-void Engine::ensure_il(int s) {
-  Side& sd = sides[s];
+void Engine::ensure_il(Side& sd) {
   if (sd.il_ok) return;
   const int64_t n = (int64_t)sd.w * sd.h;
   sd.il.ensure((size_t)n * 16);
@@ -49,8 +48,10 @@ struct Cand {
 
 SweepOut Engine::sweep(const SweepOpts& o) {
   need_images();
-  Side& A = sides[0];
-  Side& B = sides[1];
+  return sweep_sides(sides[0], sides[1], o);
+}
+
+SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   if (A.mode != B.mode) throw std::runtime_error("FeatureModeMismatch: set both images with the same smi_features");
   if (o.mode > 1) throw std::runtime_error("Unsupported: the sweep scores SMI (global or exact) only");
   const bool exact = o.mode == 1;
@@ -170,8 +171,8 @@ SweepOut Engine::sweep(const SweepOpts& o) {
   ZC_HIP(hipMemcpyAsync(sw_poses.p, poses.data(), sizeof(SweepPose) * nc, hipMemcpyHostToDevice, st));
   ZC_HIP(hipMemcpyAsync(sw_pose_fixed.p, &pf, sizeof pf, hipMemcpyHostToDevice, st));
   ZC_HIP(hipMemcpyAsync(sw_zero.p, zero.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
-  ensure_il(0);
-  ensure_il(1);
+  ensure_il(A);
+  ensure_il(B);
   const uint32_t min_n =
       (uint32_t)std::max<double>(kMinN, (double)js_round(o.min_overlap * (fw * fh) / (cell * cell)));
   CombineArgs ca{};
@@ -190,6 +191,9 @@ SweepOut Engine::sweep(const SweepOpts& o) {
     const float2* ty = twiddles((int)ny);
     // the fixed planes: its footprint at the plane origin
     SweepPack uf{(float)ox, (float)oy, (float)cell, (int)grid, B.w, B.h, (int)Fx, (int)Fy, (int)ny};
+    const double NCd = pi.n_cplx, Qd = pi.n_out, Cd = pi.n_corr, nnd = (double)nn;
+    bytes("pack_sweep_fixed", NCd * Fy * nx * 8 + 16.0 * B.w * B.h);
+    bytes("colfft_fwd_fixed", NCd * (Fy * nx * 8.0 + nnd * 8));
     LAUNCH("pack_sweep_fixed", ok &= launch_pack_sweep(plan, B.il.as<float4>(), sw_pose_fixed.as<SweepPose>(), 1, uf,
                                                         (int)nx, spec_b.as<float2>(), tx, st));
     LAUNCH("colfft_fwd_fixed", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, (int)nx, (int)ny, (int)Fy, false,
@@ -198,6 +202,14 @@ SweepOut Engine::sweep(const SweepOpts& o) {
     SweepPack um{(float)ox, (float)oy, (float)cell, (int)grid, A.w, A.h, (int)(nx - Fx + 1), (int)(ny - Fy + 1), (int)ny};
     for (uint32_t c0 = k0; c0 < k1; c0 += Bn) {
       const int b = (int)std::min(Bn, k1 - c0);
+      // algorithmic bytes of each launch: its role's minimal HBM traffic
+      bytes("pack_sweep", b * (NCd * um.win_y * nx * 8 + 16.0 * A.w * A.h));
+      bytes("colfft_fwd", b * NCd * (um.win_y * nx * 8.0 + nnd * 8));
+      bytes("cmul", b * (2 * NCd + Qd) * nnd * 8);
+      bytes("colfft_inv", b * 2 * Qd * nnd * 8);
+      bytes("rowifft_extract", b * (Qd * nnd * 8 + Cd * nnd * 4));
+      bytes("combine_sweep", b * 2 * nnd * 4);   // lower bound: n and the score of every lag
+      bytes("peaks", b * nnd * 4);
       LAUNCH("pack_sweep", ok &= launch_pack_sweep(plan, A.il.as<float4>(), dposes + c0, b, um, (int)nx,
                                                     spec_a.as<float2>(), tx, st));
       LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx * b, (int)nx, (int)ny, um.win_y, false,

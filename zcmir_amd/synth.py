@@ -11,6 +11,9 @@ them. Workloads:
 * ``rotscale_pair``: moving = fixed rotated by theta and scaled by s about the image centre
   (bilinear), then remapped by x^0.5.
 * ``stack_slices``: uneven-size slices, each with a random homography and intensity remap.
+* ``drift_stack``: square slices whose texture and pose drift slowly from slice to slice, the
+  fixed side another modality (1 - x^0.6 plus noise) - the design of the reference's stack test
+  (tests/test_api.py:186-202), re-created here.
 """
 from __future__ import annotations
 
@@ -139,4 +142,50 @@ def stack_slices(n: int = 64, side=(512, 1536), max_aspect: float = 1.5, seed: i
         name = names[int(rng.integers(len(names)))]
         moving = np.clip(REMAPS[name](np.clip(mv, 0.0, 1.0)), 0.0, 1.0)
         out.append((moving.astype(np.float32), fixed.astype(np.float32), Ht, name))
+    return out
+
+
+def rect_texture(n: int, seed: int) -> np.ndarray:
+    """An n × n texture with strong edges: smooth power-law noise under 80 randomly rotated
+    rectangles of random flat intensity, plus N(0, 0.02) noise, in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    fy, fx = np.meshgrid(np.fft.fftfreq(n), np.fft.fftfreq(n), indexing="ij")
+    r = np.hypot(fy, fx)
+    r[0, 0] = 1.0
+    spec = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * r ** -1.6
+    spec[0, 0] = 0.0
+    base = np.fft.ifft2(spec).real
+    img = 0.6 * (base - base.min()) / max(np.ptp(base), 1e-12)
+    yy, xx = np.mgrid[0:n, 0:n].astype(np.float64)
+    for _ in range(80):
+        cx, cy = rng.uniform(0, n, 2)
+        half_w, half_h = rng.uniform(6, 40, 2)
+        t = rng.uniform(0, math.pi)
+        u = (xx - cx) * math.cos(t) + (yy - cy) * math.sin(t)
+        v = (yy - cy) * math.cos(t) - (xx - cx) * math.sin(t)
+        img[(np.abs(u) < half_w) & (np.abs(v) < half_h)] = rng.uniform(0, 1)
+    return np.clip(img + 0.02 * rng.standard_normal((n, n)), 0.0, 1.0)
+
+
+def drift_stack(n_slices: int = 5, n: int = 384, seed: int = 11):
+    """[(moving, fixed, H_true)]: slice k's moving image mixes a base texture with its own
+    (85 / 15), its pose drifts (rotation -6 + 0.4 k degrees, scale 1.04 + 0.004 k, translation
+    (10 + 1.5 k, -7 + k), perspective (2e-5, -1e-5) about the centre), and the fixed image is
+    1 - moving^0.6 warped by the pose, plus N(0, 0.03), zero outside the moving image."""
+    base = rect_texture(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    c = ((n + 1) / 2.0, (n + 1) / 2.0)
+    out = []
+    for k in range(n_slices):
+        img = np.clip(0.85 * base + 0.15 * rect_texture(n, 100 + k) + 0.01 * rng.standard_normal((n, n)), 0.0, 1.0)
+        R = np.eye(3)
+        R[:2, :2] = rot(-6.0 + 0.4 * k, 1.04 + 0.004 * k)
+        R[2, :2] = (2e-5, -1e-5)
+        C = np.array([[1, 0, c[0]], [0, 1, c[1]], [0, 0, 1.0]])
+        H = np.array([[1, 0, 10.0 + 1.5 * k], [0, 1, -7.0 + k], [0, 0, 1.0]]) @ C @ R @ np.linalg.inv(C)
+        src = warp(img, H, (n, n))
+        inside = warp(np.ones_like(img), H, (n, n)) > 0.999
+        fixed = np.where(inside, 1.0 - np.clip(src, 0, 1) ** 0.6, 0.0)
+        fixed = np.clip(fixed + 0.03 * rng.standard_normal((n, n)), 0.0, 1.0)
+        out.append((img.astype(np.float32), fixed.astype(np.float32), H))
     return out
