@@ -170,6 +170,10 @@ typedef struct {                 /* search.zig:145 Peak */
  * local maxima above the floor) and nbest per-candidate best scores; returns the peak count. */
 int32_t zc_sweep(void* e, const double H0[9], zc_sweep_info* info, zc_sweep_peak* peaks, size_t cap, float* best,
                  size_t nbest);
+/* zc_search restricted to k (rotation index, scale index) cells of the sweep grid (cells: 2k
+ * indices, e.g. the learned prior's proposals); best_peak (nullable): the sweep's best raw peak
+ * score over the fixed image's area (the learned search's fallback test) */
+int32_t zc_search_cells(void* e, const uint32_t* cells, uint32_t k, zc_search_result* out, double* best_peak);
 /* the score map of the last sweep batch's first candidate; dims = (nx, ny) */
 int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
 /* candidates per sweep batch (0: 64); process-wide */
@@ -186,6 +190,39 @@ int32_t zc_stack_set(void* e, uint32_t n, const float* const* moving, const uint
  * H_init (n x 9, nullable: the poses after the map); returns the iterations run */
 int32_t zc_stack_register(void* e, const double* H0, uint32_t iters, uint32_t init_map, double* H_out,
                           double* scores, double* H_init);
+
+/* zc_set_image_gray (preprocess 0) from device memory; the work writing `data` must be complete */
+int32_t zc_set_image_device(void* e, uint32_t side, const float* data, uint32_t w, uint32_t h);
+
+/* ── the learned rotation x scale prior (no counterpart in the reference): device pointers, work
+ *    on `stream` (a hipStream_t; NULL: the null stream); 0 on success, -1 on a bad shape ── */
+/* y = relu(conv3x3(x) + b): x NHWC [n][h][w][cin], w [9*cin][cout] (ky, kx, cin major), stride 1|2,
+ * H axis periodic, W axis zero-padded; BF16 MFMA operands, FP32 accumulation */
+int32_t zc_cnn_conv_fwd(const float* x, const float* w, const float* b, float* y, int n, int h, int wd, int cin,
+                        int cout, int stride, int relu, void* stream);
+/* gradients of zc_cnn_conv_fwd from dy (through the relu via y): dx (nullable) written, dw and db
+ * accumulated */
+int32_t zc_cnn_conv_bwd(const float* x, const float* w, const float* y, const float* dy, float* dx, float* dw,
+                        float* db, int n, int h, int wd, int cin, int cout, int stride, int relu, void* stream);
+/* out[n][j] = f[n][:] . w[:][j] + b[j] */
+int32_t zc_cnn_fc_fwd(const float* f, const float* w, const float* b, float* out, int n, int k, int j, void* stream);
+/* df (nullable) written, dw and db accumulated */
+int32_t zc_cnn_fc_bwd(const float* f, const float* w, const float* dout, float* df, float* dw, float* db, int n, int k,
+                      int j, void* stream);
+/* softmax cross-entropy: loss[n], dlogits = (p - onehot) / n */
+int32_t zc_cnn_xent(const float* logits, const int* labels, float* loss, float* dlogits, int n, int j, void* stream);
+/* one Adam step (bias-corrected, step >= 1) */
+int32_t zc_cnn_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
+                    int step, void* stream);
+/* network input of `count` gray images: S x S downsample, rank, Sobel magnitude x Hann, 2-D FFT,
+ * log(1 + |F|) on T angles x R log radii (r0..r1) into channel ch of NHWC [count][T][R][C],
+ * normalized per map once ch == C - 1; work: count * S * S * 12 bytes */
+int32_t zc_lp_features(const float* imgs, int count, int w, int h, int S, int T, int R, int C, int ch, float r0,
+                       float r1, float* out, void* work, void* stream);
+/* bilinear warp of `count` images by His (count x 9, destination -> source, 1-based), zero
+ * outside, then remap (0 none, 1 1-x^2, 2 sqrt x, 3 sin pi x) */
+int32_t zc_warp_gray(const float* src, int count, int w, int h, const double* His, const int* remap, float* dst,
+                     void* stream);
 
 /* ── reference entry points outside the SMI hot path (exports.zig:95-138, 232, 245, 284-376,
  *    382, 397, 430, 458-489): exported so the reference's full ctypes symbol list binds; each

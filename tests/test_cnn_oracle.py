@@ -1,0 +1,74 @@
+"""Pinning of the prior network's oracle (CPU only): every gradient of oracle/cnn_ref.py against
+central differences of its float64 forward pass, and im2col / col2im as adjoints."""
+import numpy as np
+import pytest
+
+from oracle import cnn_ref as C
+
+LAYERS = [(2, 4, 1), (4, 4, 2), (4, 8, 2)]
+
+
+def _net(rng):
+    params = [(rng.standard_normal((9 * ci, co)) * np.sqrt(2 / (9 * ci)), rng.standard_normal(co) * 0.1)
+              for ci, co, _ in LAYERS]
+    params.append((rng.standard_normal((2 * 2 * 8, 5)) * 0.1, rng.standard_normal(5) * 0.1))
+    return params
+
+
+def test_im2col_adjoint():
+    rng = np.random.default_rng(0)
+    for s in (1, 2):
+        x = rng.standard_normal((2, 8, 6, 3))
+        c = rng.standard_normal((2 * (8 // s) * (6 // s), 27))
+        assert np.isclose(np.sum(C.im2col(x, s) * c), np.sum(x * C.col2im(c, x.shape, s)))
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 3.14159], np.float32)
+    assert np.array_equal(C.bf16(x), np.array([1.0, 1.0, 1.0 + 2 ** -7, -2.5, 3.140625], np.float32))
+
+
+def test_network_gradients_match_differences():
+    rng = np.random.default_rng(1)
+    params = _net(rng)
+    x = rng.standard_normal((2, 8, 8, 2))
+    labels = np.array([1, 3])
+
+    def loss_of(ps, xx):
+        acts, z = C.net_forward(xx, ps, LAYERS)
+        return C.xent(z, labels)[0].mean()
+
+    acts, z = C.net_forward(x, params, LAYERS)
+    _, dz = C.xent(z, labels)
+    grads, dx = C.net_backward(acts, params, LAYERS, dz)
+    h = 1e-6
+    for li, (W, b) in enumerate(params):
+        for arr, g in ((W, grads[li][0]), (b, grads[li][1])):
+            for idx in list(np.ndindex(arr.shape))[:: max(1, arr.size // 12)]:
+                old = arr[idx]
+                arr[idx] = old + h
+                lp = loss_of(params, x)
+                arr[idx] = old - h
+                lm = loss_of(params, x)
+                arr[idx] = old
+                fd = (lp - lm) / (2 * h)
+                assert g[idx] == pytest.approx(fd, rel=1e-4, abs=1e-8), (li, idx)
+    for idx in list(np.ndindex(x.shape))[::37]:
+        old = x[idx]
+        x[idx] = old + h
+        lp = loss_of(params, x)
+        x[idx] = old - h
+        lm = loss_of(params, x)
+        x[idx] = old
+        assert dx[idx] == pytest.approx((lp - lm) / (2 * h), rel=1e-4, abs=1e-8), idx
+
+
+def test_lp_features_rotation_is_a_shift():
+    """A 90° turn of the image moves its log-polar magnitude map by T/2 angle bins (exact on the
+    pixel grid); a 2× zoom out by log(2)/log(r1/r0)·R radius bins."""
+    from zcmir_amd import synth
+    img = synth.shift_pair((256, 256), (0, 0), seed=3)[1]
+    a = C.lp_features(img, S=256, T=48, R=64)
+    b = C.lp_features(np.rot90(img).copy(), S=256, T=48, R=64)
+    err = [np.abs(np.roll(a, k, axis=0) - b).mean() for k in range(48)]
+    assert int(np.argmin(err)) == 24

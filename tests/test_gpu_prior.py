@@ -1,0 +1,161 @@
+"""GPU parity of the learned prior's kernels against the NumPy oracle (oracle/cnn_ref.py): the
+BF16-MFMA dense and convolution layers forward and backward (operands rounded to bfloat16 in the
+oracle as in the kernels), softmax cross-entropy, Adam, the input features and the training
+warp. Bars: 2e-3 of each tensor's largest magnitude for the MFMA products (FP32 vs FP64
+accumulation), 1e-5 for the FP32-only kernels."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import cnn_ref as C
+from zcmir_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    return lib, torch
+
+
+def _t(torch, a):
+    return torch.tensor(np.ascontiguousarray(a, np.float32), device="cuda")
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def test_dense_layer_mfma(env):
+    """Odd sizes (none a multiple of 16 or 32) exercise the MFMA lane layout and the edges."""
+    lib, torch = env
+    rng = np.random.default_rng(0)
+    n, k, j = 37, 70, 29
+    f, w, b = rng.standard_normal((n, k)), rng.standard_normal((k, j)) * 0.1, rng.standard_normal(j)
+    tf, tw, tb = _t(torch, f), _t(torch, w), _t(torch, b)
+    out = torch.empty((n, j), device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.zc_cnn_fc_fwd(_p(tf), _p(tw), _p(tb), _p(out), n, k, j, st) == 0
+    ref = C.fc_fwd(f.astype(np.float32), w.astype(np.float32), b.astype(np.float32), bf=True)
+    assert _rel(out.cpu().numpy(), ref) < 2e-3
+    dout = rng.standard_normal((n, j))
+    td = _t(torch, dout)
+    df, dw, db = torch.empty((n, k), device="cuda"), torch.zeros((k, j), device="cuda"), torch.zeros(j, device="cuda")
+    assert lib.zc_cnn_fc_bwd(_p(tf), _p(tw), _p(td), _p(df), _p(dw), _p(db), n, k, j, st) == 0
+    rdf, rdw, rdb = C.fc_bwd(f.astype(np.float32), w.astype(np.float32), dout.astype(np.float32), bf=True)
+    assert _rel(df.cpu().numpy(), rdf) < 2e-3
+    assert _rel(dw.cpu().numpy(), rdw) < 2e-3
+    assert _rel(db.cpu().numpy(), rdb) < 1e-5
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout,stride", [(2, 8, 12, 2, 32, 1), (3, 12, 16, 32, 64, 2), (2, 6, 10, 5, 7, 2),
+                                                   (1, 48, 64, 2, 32, 1)])
+def test_conv_layer_mfma(env, n, h, w, cin, cout, stride):
+    lib, torch = env
+    rng = np.random.default_rng(cin * 100 + cout)
+    x = rng.standard_normal((n, h, w, cin)).astype(np.float32)
+    W = (rng.standard_normal((9 * cin, cout)) * np.sqrt(2 / (9 * cin))).astype(np.float32)
+    b = (rng.standard_normal(cout) * 0.1).astype(np.float32)
+    tx, tW, tb = _t(torch, x), _t(torch, W), _t(torch, b)
+    ho, wo = h // stride, w // stride
+    y = torch.empty((n, ho, wo, cout), device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.zc_cnn_conv_fwd(_p(tx), _p(tW), _p(tb), _p(y), n, h, w, cin, cout, stride, 1, st) == 0
+    yg = y.cpu().numpy()
+    assert _rel(yg, C.conv_fwd(x, W, b, stride, True, bf=True)) < 2e-3
+    dy = rng.standard_normal((n, ho, wo, cout)).astype(np.float32)
+    tdy = _t(torch, dy)
+    dx, dW, db = torch.empty_like(tx), torch.zeros_like(tW), torch.zeros_like(tb)
+    assert lib.zc_cnn_conv_bwd(_p(tx), _p(tW), _p(y), _p(tdy), _p(dx), _p(dW), _p(db), n, h, w, cin, cout, stride, 1,
+                               st) == 0
+    rdx, rdW, rdb = C.conv_bwd(x, W, yg, dy, stride, True, bf=True)   # the GPU's own activations: same mask
+    assert _rel(dx.cpu().numpy(), rdx) < 2e-3
+    assert _rel(dW.cpu().numpy(), rdW) < 2e-3
+    assert _rel(db.cpu().numpy(), rdb) < 1e-5
+
+
+def test_xent_and_adam(env):
+    lib, torch = env
+    rng = np.random.default_rng(2)
+    z = rng.standard_normal((9, 216)).astype(np.float32) * 3
+    lab = rng.integers(0, 216, 9).astype(np.int32)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    tz, tl = _t(torch, z), torch.tensor(lab, device="cuda")
+    loss, d = torch.empty(9, device="cuda"), torch.empty_like(tz)
+    assert lib.zc_cnn_xent(_p(tz), _p(tl), _p(loss), _p(d), 9, 216, st) == 0
+    rl, rd = C.xent(z.astype(np.float64), lab)
+    assert np.abs(loss.cpu().numpy() - rl).max() < 1e-5 and np.abs(d.cpu().numpy() - rd).max() < 1e-7
+    p0 = rng.standard_normal(1000).astype(np.float32)
+    g = rng.standard_normal(1000).astype(np.float32)
+    tp, tg, tm, tv = _t(torch, p0), _t(torch, g), torch.zeros(1000, device="cuda"), torch.zeros(1000, device="cuda")
+    for step in (1, 2):
+        assert lib.zc_cnn_adam(_p(tp), _p(tg), _p(tm), _p(tv), 1000, 1e-2, 0.9, 0.999, 1e-8, step, st) == 0
+    m1, v1 = 0.1 * g, 0.001 * g * g
+    p1 = p0 - 1e-2 * (m1 / 0.1) / (np.sqrt(v1 / 0.001) + 1e-8)
+    m2, v2 = 0.9 * m1 + 0.1 * g, 0.999 * v1 + 0.001 * g * g
+    p2 = p1 - 1e-2 * (m2 / (1 - 0.81)) / (np.sqrt(v2 / (1 - 0.999 ** 2)) + 1e-8)
+    assert np.abs(tp.cpu().numpy() - p2).max() < 1e-5
+
+
+def test_lp_features_match_oracle(env):
+    lib, torch = env
+    from zcmir_amd import prior
+
+    mv, fx, _ = synth.rotscale_pair((384, 320), theta_deg=25.0, scale=1.1, seed=5)
+    x = prior.features(_t(torch, mv[None]), _t(torch, fx[None])).cpu().numpy()[0]
+    for ch, img in ((0, mv), (1, fx)):
+        ref = C.lp_features(img, S=prior.S, T=prior.T, R=prior.R, r0=prior.R0, r1=prior.R1)
+        d = np.abs(x[:, :, ch] - ref)
+        assert d.mean() < 2e-3 and d.max() < 5e-2, (ch, d.mean(), d.max())
+
+
+def test_warp_gray_matches_numpy(env):
+    lib, torch = env
+    from zcmir_amd import prior
+
+    rng = np.random.default_rng(3)
+    img = synth.shift_pair((96, 96), (0, 0), seed=1)[1]
+    Hs, th, sg, rm = prior.random_poses(rng, 3, 96)
+    mv, fx = prior.make_pairs(_t(torch, img[None]), [0, 0, 0], Hs, rm)
+    got = mv.cpu().numpy()
+    remap = {1: lambda v: 1 - v * v, 2: np.sqrt, 3: lambda v: np.sin(np.pi * v)}
+    for i in range(3):
+        ref = synth.warp(img, np.linalg.inv(Hs[i]), (96, 96))   # moving(X) = fixed(H X)
+        inside = synth.warp(np.ones_like(img), np.linalg.inv(Hs[i]), (96, 96)) > 0
+        ref = np.where(inside, remap[int(rm[i])](np.clip(ref, 0, 1)), 0.0)
+        assert np.abs(got[i] - ref).max() < 1e-4
+
+
+def test_training_steps_reduce_the_loss(env):
+    """A few hundred sweep-labelled pairs: the labels mostly match the true cells, and Adam steps
+    on the project's kernels reduce the training loss."""
+    lib, torch = env
+    import zcmir_amd
+    from zcmir_amd import prior
+
+    rng = np.random.default_rng(4)
+    tex = torch.tensor(np.stack([synth._texture(256, 256, rng) for _ in range(8)]).astype(np.float32), device="cuda")
+    Hs, th, sg, rm = prior.random_poses(rng, 160, 256, shift=8.0)
+    mv, fx = prior.make_pairs(tex, rng.integers(0, 8, 160), Hs, rm)
+    reg = zcmir_amd.Registrar(half=False)
+    lab, _ = prior.sweep_labels(reg, mv, fx, grid=64)
+    reg.close()
+    agree = np.mean([prior.truth_cell(th[i], sg[i]) == (lab[i] // 9, lab[i] % 9) for i in range(160)])
+    assert agree > 0.6, agree
+    x = prior.features(mv, fx)
+    y = torch.tensor(lab, device="cuda")
+    net = prior.Net(0)
+    first = float(net.step(x, y, 1e-3))
+    for _ in range(40):
+        last = float(net.step(x, y, 1e-3))
+    assert last < 0.7 * first, (first, last)

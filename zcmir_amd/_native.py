@@ -12,6 +12,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 
 U32, I32, I64, F64, SZ, P = ctypes.c_uint32, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
+F32 = ctypes.c_float
 f32p, f64p, u8p, u32p = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint32)
 Mat = ctypes.c_double * 9
 
@@ -139,6 +140,17 @@ _SIGS = {
     "zc_sweep_tune": ([U32], None),
     "zc_stack_set": ([P, U32, P, u32p, P, u32p], I32),
     "zc_stack_register": ([P, f64p, U32, U32, f64p, f64p, f64p], I32),
+    "zc_set_image_device": ([P, U32, P, U32, U32], I32),
+    "zc_search_cells": ([P, u32p, U32, P, f64p], I32),
+    # the learned prior (device pointers as integers, a HIP stream handle)
+    "zc_cnn_conv_fwd": ([P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P], I32),
+    "zc_cnn_conv_bwd": ([P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P], I32),
+    "zc_cnn_fc_fwd": ([P, P, P, P, I32, I32, I32, P], I32),
+    "zc_cnn_fc_bwd": ([P, P, P, P, P, P, I32, I32, I32, P], I32),
+    "zc_cnn_xent": ([P, P, P, P, I32, I32, P], I32),
+    "zc_cnn_adam": ([P, P, P, P, I64, F32, F32, F32, F32, I32, P], I32),
+    "zc_lp_features": ([P, I32, I32, I32, I32, I32, I32, I32, I32, F32, F32, P, P, P], I32),
+    "zc_warp_gray": ([P, I32, I32, I32, P, P, P, P], I32),
 }
 
 LIB_NAME = "libzcsmi.so"

@@ -145,6 +145,12 @@ int32_t zc_set_image_gray(void* p, uint32_t side, const float* data, uint32_t w,
   return guard(e, "set_image_gray", [&] { e->set_image((int)side, data, (int)w, (int)h, preprocess != 0); });
 }
 
+// zc_set_image_gray from device memory (work that writes `data` must be complete).
+int32_t zc_set_image_device(void* p, uint32_t side, const float* data, uint32_t w, uint32_t h) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "set_image_device", [&] { e->set_image((int)side, data, (int)w, (int)h, false); });
+}
+
 int32_t zc_set_image_rgba(void* p, uint32_t side, const uint8_t* rgba, uint32_t w, uint32_t h) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "set_image_rgba", [&] {
@@ -482,6 +488,30 @@ int32_t zc_stack_register(void* p, const double* H0, uint32_t iters, uint32_t in
     }
   });
   return rc < 0 ? rc : (int32_t)it;
+}
+
+// zc_search over `k` given (rotation index, scale index) cells of the sweep grid only (cells:
+// 2·k indices; the learned prior's proposals); best_peak (nullable): the sweep's best raw peak
+// score divided by the fixed image's area.
+int32_t zc_search_cells(void* p, const uint32_t* cells, uint32_t k, zc::SearchResult* out, double* best_peak) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "search_cells", [&] {
+    e->need_images();
+    e->cancel = false;
+    const auto& s = e->set;
+    zc::SweepGrid gv;
+    zc::SweepOpts o = sweep_opts(e, gv);
+    std::vector<std::array<uint32_t, 2>> cl(k);
+    for (uint32_t i = 0; i < k; ++i) cl[i] = {cells[2 * i], cells[2 * i + 1]};
+    o.cells = &cl;
+    const uint32_t kk = s.u("sw_k");
+    const uint32_t keep = (kk == 8 || kk == 16 || kk == 32) ? kk : 32;
+    const uint32_t n_fin = std::min<uint32_t>(std::max<uint32_t>(s.u("sw_fin"), 1), 16);
+    const auto amps = zc::geo_amps(s.f("g_a0"), s.f("g_decay"), s.u("g_steps") > 0 ? s.u("g_steps") : 7);
+    double bp = 0;
+    *out = e->search(o, keep, n_fin, s.e("group") == "homography", s.b("exact"), s.b("symmetric"), amps, &bp);
+    if (best_peak) *best_peak = bp / ((double)e->sides[1].w * e->sides[1].h);
+  });
 }
 
 // The score map of the last sweep batch's first candidate (nx × ny into dims).

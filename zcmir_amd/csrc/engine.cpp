@@ -143,7 +143,8 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
   sd.feat.ensure(n * 16);
   sd.mag.ensure(n * 4);
   sd.stack.ensure(n * 20);
-  ZC_HIP(hipMemcpyAsync(sd.gray.p, data, n * 4, hipMemcpyHostToDevice, st));
+  // host or device memory (the caller finishes any device work that writes `data` first)
+  ZC_HIP(hipMemcpyAsync(sd.gray.p, data, n * 4, hipMemcpyDefault, st));
   prepare(sd, true);
   sd.has = true;
 }

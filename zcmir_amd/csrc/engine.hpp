@@ -163,6 +163,8 @@ struct SweepOpts {
   uint32_t n_peak = 4;
   uint32_t batch = 64;
   double min_overlap = 0.05;
+  // only these (rotation, scale) index pairs of the grid (shear and stretch index 0), in order
+  const std::vector<std::array<uint32_t, 2>>* cells = nullptr;
 };
 
 struct SweepOut {
@@ -263,8 +265,9 @@ class Engine {
                               bool homography, bool exact, bool symmetric, const std::vector<double>& amps,
                               uint32_t* iterations = nullptr);
   std::vector<PairSides> own_pair(size_t n) const;
+  // best_peak: the sweep's best raw peak score (n in fixed pixels), nullable
   SearchResult search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, bool homography, bool exact,
-                      bool symmetric, const std::vector<double>& amps);
+                      bool symmetric, const std::vector<double>& amps, double* best_peak = nullptr);
   // stacks (stack.cpp): slice pairs with their own features, registered together
   std::vector<std::unique_ptr<Side>> stack;   // moving, fixed per slice
   void stack_set(uint32_t n, const float* const* mv, const uint32_t* mwh, const float* const* fx,

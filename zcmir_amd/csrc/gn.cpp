@@ -428,7 +428,7 @@ std::vector<PairSides> Engine::own_pair(size_t n) const { return std::vector<Pai
 // the peaks → 25 iterations on the finalists (the best screened poses and the best coarse
 // peaks); the best pose replaces the current one when it beats its score.
 SearchResult Engine::search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, bool homography, bool exact,
-                            bool symmetric, const std::vector<double>& amps) {
+                            bool symmetric, const std::vector<double>& amps, double* best_peak) {
   need_images();
   const Mat3 Hcur = H;
   const double s_cur = pair_scores(own_pair(1), {Hcur}, exact, symmetric)[0];
@@ -436,6 +436,10 @@ SearchResult Engine::search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, b
   for (int i = 0; i < 9; ++i) out.H[i] = Hcur[i];
   out.score = out.prev = s_cur;
   const SweepOut sw = sweep(o);
+  if (best_peak) {
+    *best_peak = 0;
+    for (const SweepPeak& p : sw.peaks) *best_peak = std::max(*best_peak, p.score);
+  }
   const std::vector<SweepPeak> top = nms4(sw.peaks, (uint32_t)o.gv->thetas.size(), keep, o.gv->wrap);
   out.count_a = (uint32_t)top.size();
   if (top.empty()) return out;

@@ -8,6 +8,11 @@
 // row IFFT + extract → FP64 per-lag whitened score + block argmax → argmax finish.
 // Pose moments: the 45 overlap sums at a pose (FP32 two-sum compensated per thread, FP64 across).
 #include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
 #include <hip/hip_fp16.h>
 #include <cstdint>
 #include <cmath>
@@ -1451,6 +1456,32 @@ bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw
   ZC_POW2_SWITCH(n, CALL)
 #undef CALL
   return true;
+}
+
+// Twiddles e^(−2πi t / L), FP64-computed, one table per (device, L) for the life of the process.
+static const float2* static_twiddles(int L) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, float2*> tables;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = tables.find({dev, L});
+  if (it != tables.end()) return it->second;
+  std::vector<float2> h(L);
+  for (int t = 0; t < L; ++t) {
+    const double a = -2.0 * M_PI * (double)t / (double)L;
+    h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * L) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  tables[{dev, L}] = d;
+  return d;
+}
+
+bool launch_fft2_twiddled(float2* data, int planes, int n, bool inverse, hipStream_t st) {
+  const float2* tw = static_twiddles(n);
+  return tw && launch_fft2(data, planes, n, inverse, tw, st, 0);
 }
 
 int moments_blocks(int w, int h) {

@@ -64,6 +64,8 @@ void launch_argmax_finish(const float* arg_part, int nparts, const float* map, f
 
 // plain 2-D FFT of `planes` n × n planes (tests, benchmarks)
 bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st, int lines = 0);
+// the same with a process-wide twiddle table per length
+bool launch_fft2_twiddled(float2* data, int planes, int n, bool inverse, hipStream_t st);
 
 // ── pose moments (45 sums per pose, per-block partials) ──
 int moments_blocks(int w, int h);

@@ -75,38 +75,49 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   const int64_t G = grid;
   const auto fcx = cells_of(0.5, fw + 0.5, ox, cell, G), fcy = cells_of(0.5, fh + 0.5, oy, cell, G);
   const uint32_t Fx = fcx[1] - fcx[0], Fy = fcy[1] - fcy[0];
-  // candidates, θ-major
-  const uint32_t nc = (uint32_t)gv.count();
+  // candidates, θ-major (or the given cells, in order)
+  std::vector<std::array<uint32_t, 4>> idx;
+  if (o.cells) {
+    for (const auto& c : *o.cells) {
+      if (c[0] >= gv.thetas.size() || c[1] >= gv.sigmas.size()) throw std::runtime_error("BadArgument: sweep cell");
+      idx.push_back({c[0], c[1], 0, 0});
+    }
+  } else {
+    for (uint32_t i = 0; i < gv.thetas.size(); ++i)
+      for (uint32_t j = 0; j < gv.sigmas.size(); ++j)
+        for (uint32_t ik = 0; ik < gv.shears.size(); ++ik)
+          for (uint32_t ia = 0; ia < gv.anisos.size(); ++ia) idx.push_back({i, j, ik, ia});
+  }
+  const uint32_t nc = (uint32_t)idx.size();
+  if (nc == 0) throw std::runtime_error("BadArgument: no sweep candidates");
   std::vector<Cand> cands(nc);
   const double s0_raw = std::sqrt(std::fabs(o.H0[0] * o.H0[4] - o.H0[1] * o.H0[3]));
   const double s0 = (s0_raw == 0 || std::isnan(s0_raw)) ? 1 : s0_raw;
   {
     size_t ci = 0;
-    for (uint32_t i = 0; i < gv.thetas.size(); ++i)
-      for (uint32_t j = 0; j < gv.sigmas.size(); ++j)
-        for (uint32_t ik = 0; ik < gv.shears.size(); ++ik)
-          for (uint32_t ia = 0; ia < gv.anisos.size(); ++ia) {
-            const auto L = cand_linear(gv.thetas[i], gv.sigmas[j], gv.shears[ik], gv.anisos[ia]);
-            const Mat3 H = mul3(about_c(cmv, L), o.H0);
-            double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
-            const double qs[4][2] = {{0.5, 0.5}, {mw + 0.5, 0.5}, {mw + 0.5, mh + 0.5}, {0.5, mh + 0.5}};
-            for (const auto& q : qs) {
-              const auto p = map_pt(H, q[0], q[1]);
-              x0 = std::min(x0, p[0]);
-              x1 = std::max(x1, p[0]);
-              y0 = std::min(y0, p[1]);
-              y1 = std::max(y1, p[1]);
-            }
-            const bool ok = std::isfinite(x0 + x1 + y0 + y1);
-            const auto mcx = ok ? cells_of(x0, x1, ox, cell, G) : std::array<uint32_t, 2>{0, grid};
-            const auto mcy = ok ? cells_of(y0, y1, oy, cell, G) : std::array<uint32_t, 2>{0, grid};
-            Cand& c = cands[ci++];
-            c = {i, j, ik, ia, gv.thetas[i], gv.sigmas[j], H, mcx[0], mcy[0],
-                 plane_len(mcx[1] - mcx[0] + Fx - 1), plane_len(mcy[1] - mcy[0] + Fy - 1), 1};
-            // box-filter taps per cell side from the candidate's smallest stretch
-            c.taps = (int)std::min(6.0, std::max(1.0, std::ceil(cell / (sv2(L)[1] * s0) - 1e-3)));
-            if (c.nx > (uint32_t)kMaxFft || c.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
-          }
+    for (const auto& q : idx) {
+      const uint32_t i = q[0], j = q[1], ik = q[2], ia = q[3];
+      const auto L = cand_linear(gv.thetas[i], gv.sigmas[j], gv.shears[ik], gv.anisos[ia]);
+      const Mat3 H = mul3(about_c(cmv, L), o.H0);
+      double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+      const double qs[4][2] = {{0.5, 0.5}, {mw + 0.5, 0.5}, {mw + 0.5, mh + 0.5}, {0.5, mh + 0.5}};
+      for (const auto& cq : qs) {
+        const auto p = map_pt(H, cq[0], cq[1]);
+        x0 = std::min(x0, p[0]);
+        x1 = std::max(x1, p[0]);
+        y0 = std::min(y0, p[1]);
+        y1 = std::max(y1, p[1]);
+      }
+      const bool ok = std::isfinite(x0 + x1 + y0 + y1);
+      const auto mcx = ok ? cells_of(x0, x1, ox, cell, G) : std::array<uint32_t, 2>{0, grid};
+      const auto mcy = ok ? cells_of(y0, y1, oy, cell, G) : std::array<uint32_t, 2>{0, grid};
+      Cand& c = cands[ci++];
+      c = {i, j, ik, ia, gv.thetas[i], gv.sigmas[j], H, mcx[0], mcy[0],
+           plane_len(mcx[1] - mcx[0] + Fx - 1), plane_len(mcy[1] - mcy[0] + Fy - 1), 1};
+      // box-filter taps per cell side from the candidate's smallest stretch
+      c.taps = (int)std::min(6.0, std::max(1.0, std::ceil(cell / (sv2(L)[1] * s0) - 1e-3)));
+      if (c.nx > (uint32_t)kMaxFft || c.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
+    }
   }
   // slots grouped by plane size; perm[k] = the candidate in slot k
   std::vector<uint32_t> perm(nc);

@@ -1,0 +1,339 @@
+"""The learned rotation × scale prior: a small CNN that proposes where the sweep should look.
+
+Each image of a pair becomes a log-polar map of its Fourier magnitude (zc_lp_features: 256²
+downsample, rank, Sobel magnitude, FFT, 48 angles over [0, π) × 64 log radii over 4 .. 128
+cycles): translation drops out, a rotation of the image is a periodic shift along the angle
+axis and a scale change a shift along the radius axis. The two maps (moving, fixed) are the two
+channels of the network's input; six 3×3 convolutions (32 – 64 channels, ReLU, three of stride
+2) and a dense head give logits over 24 rotations (15° steps) × 9 scales (0.8 – 1.25, log-spaced).
+The magnitude spectrum cannot tell θ from θ + 180°, so the network spreads its belief over both;
+the SMI scores of the proposed cells decide.
+
+Every layer runs on this library's own HIP kernels (BF16 MFMA, FP32 accumulation) for both the
+forward and the backward pass; PyTorch only holds the device buffers. Training pairs are warped
+on the GPU by zc_warp_gray and labelled by the sweep's own argmax over the same 24 × 9 grid.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+LAYERS = [(2, 32, 1), (32, 32, 2), (32, 64, 1), (64, 64, 2), (64, 64, 1), (64, 64, 2)]
+T, R, C = 48, 64, 2            # angles, log radii, channels of the input map
+S = 256                        # the images' downsampled side
+R0, R1 = 4.0, 128.0            # radius range (cycles per 256 px)
+N_TH, N_SG = 24, 9             # label grid: 24 rotations × 9 scales
+THETAS = [-180.0 + 15.0 * k for k in range(N_TH)]
+SIGMAS = [1.25 ** ((k - 4) / 4) for k in range(N_SG)]
+WEIGHTS = Path(__file__).resolve().parent / "prior_weights.npz"
+
+
+def _lib():
+    from . import _load
+    return _load()
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: kernel launch failed (rc {rc})")
+
+
+class Net:
+    """Parameters (FP32 masters on the GPU), Adam state, forward and backward on HIP kernels."""
+
+    def __init__(self, seed: int = 0):
+        import torch
+
+        rng = np.random.default_rng(seed)
+        self.shapes = []
+        h, w = T, R
+        params = []
+        for cin, cout, s in LAYERS:
+            params.append(rng.standard_normal((9 * cin, cout)) * math.sqrt(2.0 / (9 * cin)))
+            params.append(np.zeros(cout))
+            h, w = h // s, w // s
+        self.feat_hw = (h, w)
+        k = h * w * LAYERS[-1][1]
+        params.append(rng.standard_normal((k, N_TH * N_SG)) * math.sqrt(1.0 / k))
+        params.append(np.zeros(N_TH * N_SG))
+        self.p = [torch.tensor(a, dtype=torch.float32, device="cuda") for a in params]
+        self.g = [torch.zeros_like(t) for t in self.p]
+        self.m = [torch.zeros_like(t) for t in self.p]
+        self.v = [torch.zeros_like(t) for t in self.p]
+        self.t = 0
+
+    # ── persistence ──
+    def save(self, path=WEIGHTS, **meta):
+        arrs = {f"p{i}": t.detach().cpu().numpy() for i, t in enumerate(self.p)}
+        np.savez_compressed(path, **arrs, **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path=WEIGHTS):
+        import torch
+
+        net = cls()
+        with np.load(path, allow_pickle=False) as z:
+            for i in range(len(net.p)):
+                net.p[i] = torch.tensor(z[f"p{i}"], dtype=torch.float32, device="cuda")
+            net.meta = {k: z[k] for k in z.files if not k.startswith("p")}
+        return net
+
+    # ── passes ──
+    def forward(self, x):
+        """x: [n, T, R, C] float32 CUDA tensor. Returns (activations, logits [n, 216])."""
+        import torch
+
+        lib, st = _lib(), _stream()
+        n = x.shape[0]
+        acts = [x.contiguous()]
+        h, w = T, R
+        for i, (cin, cout, s) in enumerate(LAYERS):
+            y = torch.empty((n, h // s, w // s, cout), dtype=torch.float32, device="cuda")
+            _ok(lib.zc_cnn_conv_fwd(_p(acts[-1]), _p(self.p[2 * i]), _p(self.p[2 * i + 1]), _p(y), n, h, w, cin, cout,
+                                    s, 1, st), "conv_fwd")
+            acts.append(y)
+            h, w = h // s, w // s
+        logits = torch.empty((n, N_TH * N_SG), dtype=torch.float32, device="cuda")
+        k = h * w * LAYERS[-1][1]
+        _ok(lib.zc_cnn_fc_fwd(_p(acts[-1]), _p(self.p[-2]), _p(self.p[-1]), _p(logits), n, k, N_TH * N_SG, st),
+            "fc_fwd")
+        return acts, logits
+
+    def backward(self, acts, dlogits):
+        """Gradients into self.g from d(loss)/d(logits)."""
+        import torch
+
+        lib, st = _lib(), _stream()
+        for g in self.g:
+            g.zero_()
+        n = dlogits.shape[0]
+        f = acts[-1]
+        k = f[0].numel()
+        d = torch.empty_like(f)
+        _ok(lib.zc_cnn_fc_bwd(_p(f), _p(self.p[-2]), _p(dlogits), _p(d), _p(self.g[-2]), _p(self.g[-1]), n, k,
+                              N_TH * N_SG, st), "fc_bwd")
+        for i in range(len(LAYERS) - 1, -1, -1):
+            cin, cout, s = LAYERS[i]
+            x = acts[i]
+            h, w = x.shape[1], x.shape[2]
+            dx = torch.empty_like(x) if i > 0 else None
+            _ok(lib.zc_cnn_conv_bwd(_p(x), _p(self.p[2 * i]), _p(acts[i + 1]), _p(d),
+                                    _p(dx) if dx is not None else None, _p(self.g[2 * i]), _p(self.g[2 * i + 1]), n, h,
+                                    w, cin, cout, s, 1, st), "conv_bwd")
+            d = dx
+
+    def loss(self, logits, labels):
+        """(per-sample losses, d(mean)/d(logits)) on the GPU."""
+        import torch
+
+        n = logits.shape[0]
+        loss = torch.empty(n, dtype=torch.float32, device="cuda")
+        d = torch.empty_like(logits)
+        _ok(_lib().zc_cnn_xent(_p(logits), _p(labels), _p(loss), _p(d), n, N_TH * N_SG, _stream()), "xent")
+        return loss, d
+
+    def step(self, x, labels, lr=1e-3):
+        """One Adam step on a batch; returns the mean loss (a device scalar)."""
+        acts, z = self.forward(x)
+        loss, d = self.loss(z, labels)
+        self.backward(acts, d)
+        self.t += 1
+        lib, st = _lib(), _stream()
+        for p, g, m, v in zip(self.p, self.g, self.m, self.v):
+            _ok(lib.zc_cnn_adam(_p(p), _p(g), _p(m), _p(v), p.numel(), lr, 0.9, 0.999, 1e-8, self.t, st), "adam")
+        return loss.mean()
+
+
+def features(moving, fixed):
+    """Network input [n, T, R, 2] from CUDA tensors moving, fixed [n, h, w] (gray, [0, 1])."""
+    import torch
+
+    n, h, w = moving.shape
+    out = torch.empty((n, T, R, C), dtype=torch.float32, device="cuda")
+    work = torch.empty(n * S * S * 3, dtype=torch.float32, device="cuda")
+    lib, st = _lib(), _stream()
+    for ch, img in ((0, moving), (1, fixed)):
+        hh, ww = img.shape[1], img.shape[2]
+        _ok(lib.zc_lp_features(_p(img.contiguous()), n, ww, hh, S, T, R, C, ch, R0, R1, _p(out), _p(work), st),
+            "lp_features")
+    return out
+
+
+def propose(net, moving, fixed, k=8):
+    """The k most likely (rotation index, scale index) cells for each pair, best first."""
+    import torch
+
+    _, z = net.forward(features(moving, fixed))
+    top = torch.topk(z, k, dim=1).indices.cpu().numpy()
+    return [[(int(c) // N_SG, int(c) % N_SG) for c in row] for row in top]
+
+
+# ── training data: pairs the method warps and labels ──────────────────────────────────────
+def _about(c, A):
+    T_ = np.eye(3)
+    T_[:2, :2] = A
+    return np.array([[1, 0, c[0]], [0, 1, c[1]], [0, 0, 1.0]]) @ T_ @ np.array([[1, 0, -c[0]], [0, 1, -c[1]], [0, 0, 1]])
+
+
+def random_poses(rng, n, size, shift=16.0):
+    """n candidate-convention poses (moving → fixed) about the centre of size × size images:
+    rotation uniform over the circle, scale log-uniform over 0.8 – 1.25, a shift of up to
+    `shift` px; with the remap of each pair (1: 1 − x², 2: √x, 3: sin πx)."""
+    c = ((size + 1) / 2.0, (size + 1) / 2.0)
+    th = rng.uniform(-180.0, 180.0, n)
+    sg = np.exp(rng.uniform(math.log(0.8), math.log(1.25), n))
+    t = rng.uniform(-shift, shift, (n, 2))
+    Hs = []
+    for i in range(n):
+        a = math.radians(th[i])
+        A = sg[i] * np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+        Hs.append(np.array([[1, 0, t[i, 0]], [0, 1, t[i, 1]], [0, 0, 1.0]]) @ _about(c, A))
+    return np.stack(Hs), th, sg, rng.integers(1, 4, n)
+
+
+def truth_cell(th, sg):
+    """The label grid cell nearest a rotation (degrees) and scale."""
+    it = int(round((((th + 180.0) % 360.0)) / 15.0)) % N_TH
+    is_ = int(min(max(round(4 * math.log(sg) / math.log(1.25) + 4), 0), N_SG - 1))
+    return it, is_
+
+
+def make_pairs(textures, idx, Hs, remaps):
+    """moving[i] = remap(fixed[i] at H_i X): fixed = textures[idx] (CUDA [k, s, s])."""
+    import torch
+
+    fixed = textures[torch.as_tensor(idx, device="cuda")].contiguous()
+    n, s, _ = fixed.shape
+    His = torch.tensor(np.asarray(Hs).reshape(n, 9), dtype=torch.float64, device="cuda")
+    rm = torch.tensor(np.asarray(remaps, np.int32), device="cuda")
+    moving = torch.empty_like(fixed)
+    _ok(_lib().zc_warp_gray(_p(fixed), n, s, s, _p(His), _p(rm), _p(moving), _stream()), "warp_gray")
+    return moving, fixed
+
+
+def sweep_labels(reg, moving, fixed, grid=64):
+    """The sweep's argmax cell (24 × 9 grid) for each pair: the label of the pair, and the cell's
+    best SMI score divided by the fixed image's area."""
+    import torch
+
+    torch.cuda.synchronize()
+    reg.configure(sw_nth=N_TH, sw_smin=0.8, sw_smax=1.25, sw_ns=N_SG, sw_grid=grid, sw_rot_range=False, sw_rot=180,
+                  sw_full=False)
+    lib = reg._lib
+    labels, best_norm = [], []
+    n, h, w = fixed.shape
+    for i in range(n):
+        reg._check(lib.zc_set_image_device(reg._e, 0, _p(moving[i]), w, h), "set_image_device")
+        reg._check(lib.zc_set_image_device(reg._e, 1, _p(fixed[i]), w, h), "set_image_device")
+        _, _, best = reg.sweep()
+        k = int(np.argmax(best))
+        labels.append(k)
+        best_norm.append(float(best[k]) / (w * h))
+    return np.array(labels, np.int32), np.array(best_norm)
+
+
+def train(n_pairs=16000, n_val=1000, size=512, epochs=12, batch=128, lr=1e-3, seed=1, log=print, out=WEIGHTS):
+    """Generate and label pairs on the GPU, train the network, save WEIGHTS. Returns metrics."""
+    import time
+
+    import torch
+
+    import zcmir_amd
+    from . import synth
+
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    tex = np.stack([synth._texture(size, size, rng) for _ in range(96)]).astype(np.float32)
+    textures = torch.tensor(tex, device="cuda")
+    log(f"textures {time.time() - t0:.1f} s")
+    reg = zcmir_amd.Registrar(half=False)
+    total = n_pairs + n_val
+    Hs, th, sg, rm = random_poses(rng, total, size)
+    idx = rng.integers(0, len(tex), total)
+    X, Y, agree, norms = [], [], [], []
+    for s0 in range(0, total, 500):
+        s1 = min(total, s0 + 500)
+        mv, fx = make_pairs(textures, idx[s0:s1], Hs[s0:s1], rm[s0:s1])
+        lab, bn = sweep_labels(reg, mv, fx)
+        X.append(features(mv, fx))
+        Y.append(lab)
+        norms.append(bn)
+        agree += [truth_cell(th[i], sg[i]) == (lab[i - s0] // N_SG, lab[i - s0] % N_SG) for i in range(s0, s1)]
+        log(f"pairs {s1}/{total}  {time.time() - t0:.1f} s  label = nearest true cell: {np.mean(agree):.3f}")
+    reg.close()
+    X = torch.cat(X)
+    Y = torch.tensor(np.concatenate(Y), device="cuda")
+    norms = np.concatenate(norms)
+    net = Net(seed)
+    xtr, ytr, xva, yva = X[:n_pairs], Y[:n_pairs], X[n_pairs:], Y[n_pairs:]
+    steps = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n_pairs, device="cuda")
+        run = 0.0
+        for b0 in range(0, n_pairs - batch + 1, batch):
+            sel = perm[b0:b0 + batch]
+            lr_t = lr * 0.5 * (1 + math.cos(math.pi * steps / (epochs * (n_pairs // batch))))
+            run += float(net.step(xtr[sel], ytr[sel], lr_t))
+            steps += 1
+        acc = evaluate(net, xva, yva)
+        log(f"epoch {ep + 1}: loss {run / (n_pairs // batch):.3f}  val top1 {acc['top1']:.3f} top8 {acc['top8']:.3f} "
+            f"top8±1θ {acc['top8_near']:.3f}  {time.time() - t0:.1f} s")
+    thr = float(np.quantile(norms[n_pairs:], 0.05))
+    net.save(out, threshold=thr, label_agreement=np.mean(agree), n_pairs=n_pairs, size=size)
+    return dict(evaluate(net, xva, yva), threshold=thr, label_agreement=float(np.mean(agree)),
+                seconds=time.time() - t0)
+
+
+def evaluate(net, x, y, k=8):
+    """Top-1, top-k and top-k-within-one-rotation-step hit rates (θ and θ + 180° count as one)."""
+    import torch
+
+    with torch.no_grad():
+        _, z = net.forward(x)
+    top = torch.topk(z, k, dim=1).indices.cpu().numpy()
+    lab = y.cpu().numpy()
+
+    def same(c, l, near):
+        dt = abs(c // N_SG - l // N_SG) % 12
+        dt = min(dt, 12 - dt)
+        return c % N_SG == l % N_SG and dt <= near
+
+    t1 = np.mean([same(top[i, 0], lab[i], 0) for i in range(len(lab))])
+    tk = np.mean([any(same(c, lab[i], 0) for c in top[i]) for i in range(len(lab))])
+    tn = np.mean([any(same(c, lab[i], 1) for c in top[i]) for i in range(len(lab))])
+    return dict(top1=float(t1), top8=float(tk), top8_near=float(tn))
+
+
+def main():
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train the learned rotation × scale prior on the GPU")
+    ap.add_argument("--pairs", type=int, default=16000)
+    ap.add_argument("--val", type=int, default=1000)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=str(WEIGHTS))
+    a = ap.parse_args()
+    r = train(a.pairs, a.val, a.size, a.epochs, a.batch, a.lr, log=lambda s: print(s, flush=True), out=Path(a.out))
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
