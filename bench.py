@@ -135,6 +135,78 @@ def bench_sweep(a, np, torch, zcmir_amd, synth):
     return out
 
 
+def _pose_close(Ha, Hb, w, h):
+    """Corner distance (px) and rotation difference (degrees) of two poses."""
+    import numpy as np
+
+    pts = np.array([[1, 1, 1], [w, 1, 1], [1, h, 1], [w, h, 1]], float).T
+    a, b = Ha @ pts, Hb @ pts
+    dist = float(np.abs(a[:2] / a[2] - b[:2] / b[2]).max())
+    rot = lambda H: np.degrees(np.arctan2(H[1, 0] - H[0, 1], H[0, 0] + H[1, 1]))
+    dth = abs((rot(Ha) - rot(Hb) + 180) % 360 - 180)
+    return dist, float(dth)
+
+
+def bench_prior(a, np, torch, zcmir_amd):
+    """Time to pose, learned prior vs the full sweep: 1024² pairs made as the rotation × scale
+    workload with a random pose (rotation over the circle, scale 0.8 – 1.25, shift ≤ 16 px) and a
+    random remap, held out from training. Full: the sweep search (72 θ × 9 σ, NMS, Gauss–Newton).
+    Learned: the network's top-8 (rotation mod 180°, scale) classes → the sweep over the 48 grid
+    cells inside their 16 cells → NMS, Gauss–Newton; the full search runs when the proposals' best SMI (128² canvas) falls below
+    0.8 × the threshold held out from training. Both at full resolution (one cell per fixed
+    pixel)."""
+    from zcmir_amd import prior, synth
+
+    if not prior.WEIGHTS.exists():
+        return {"error": f"{prior.WEIGHTS.name} missing"}
+    net = prior.Net.load()
+    rng = np.random.default_rng(20251)
+    n_eval, n_cal = a.prior_pairs, 8
+    size = 1024
+    tex = torch.tensor(np.stack([synth._texture(size, size, rng) for _ in range(n_eval + n_cal)]).astype(np.float32),
+                       device="cuda")
+    Hs, th, sg, rm = prior.random_poses(rng, n_eval + n_cal, size)
+    mv, fx = prior.make_pairs(tex, np.arange(n_eval + n_cal), Hs, rm)
+    torch.cuda.synchronize()
+    reg = zcmir_amd.Registrar(half=False, sw_full=True, **SWEEP_GRID)
+    # the fallback threshold, from held-out pairs (the last n_cal) at their true cells
+    thr = prior.calibrate(reg, net, mv[n_eval:], fx[n_eval:], Hs[n_eval:], th[n_eval:], sg[n_eval:])
+    # warm both paths once (first-use allocations and twiddle tables)
+    reg.set_images(mv[0].cpu().numpy(), fx[0].cpu().numpy(), preprocess=False)
+    prior.learned_search(reg, net, mv[0], fx[0], threshold=thr)
+    rows = []
+    for i in range(n_eval):
+        m_np, f_np = mv[i].cpu().numpy(), fx[i].cpu().numpy()
+        reg.set_images(m_np, f_np, preprocess=False)
+        reg.pose = np.eye(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        full = reg.search()
+        t_full = time.perf_counter() - t0
+        reg.pose = np.eye(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lrn, info = prior.learned_search(reg, net, mv[i], fx[i], threshold=thr)
+        torch.cuda.synchronize()
+        t_lrn = time.perf_counter() - t0
+        d, dth = _pose_close(lrn.H, full.H, size, size)
+        rows.append({"t_full_s": round(t_full, 4), "t_learned_s": round(t_lrn, 4), "pose_dist_px": round(d, 4),
+                     "rot_diff_deg": round(dth, 4), "fell_back": bool(info["fell_back"]),
+                     "check_score": round(info["check_score"], 4),
+                     "truth_cell_proposed": prior.truth_cell(th[i], sg[i]) in info["cells"],
+                     "truth_dist_px": round(_pose_close(full.H, Hs[i], size, size)[0], 3)})
+    reg.close()
+    tf = float(np.mean([r["t_full_s"] for r in rows]))
+    tl = float(np.mean([r["t_learned_s"] for r in rows]))
+    same = [r["pose_dist_px"] <= 0.5 and r["rot_diff_deg"] <= 0.5 for r in rows]
+    return {"metric": "time-to-pose learned vs full sweep", "value": round(tf / tl, 2), "unit": "x",
+            "time_to_pose_full_s": round(tf, 4), "time_to_pose_learned_s": round(tl, 4),
+            "same_pose_rate": float(np.mean(same)), "fallbacks": int(sum(r["fell_back"] for r in rows)),
+            "threshold": thr, "pairs": rows,
+            "network": {"layers": prior.LAYERS, "input": [prior.T, prior.R, prior.C],
+                        "trained_pairs": int(net.meta.get("n_pairs", 0))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +219,7 @@ def main():
     ap.add_argument("--fft-cols", type=int, default=0, help="columns per workgroup of the column FFT (0: default)")
     ap.add_argument("--sweep-steps", type=int, default=10, help="timed sweeps on the 128² canvas (0: skip)")
     ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
+    ap.add_argument("--prior-pairs", type=int, default=6, help="time-to-pose pairs, learned vs full (0: skip)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -254,6 +327,8 @@ def main():
                          "scaled 1.12 (bilinear), remapped x^0.5; 72 rotations x 9 scales (0.8-1.25) = 648 "
                          "candidates, exact SMI, top-4 local maxima per candidate map",
                          **bench_sweep(a, np, torch, zcmir_amd, synth)}
+    if a.prior_pairs > 0:
+        line["prior"] = bench_prior(a, np, torch, zcmir_amd)
     print(json.dumps(line))
 
 

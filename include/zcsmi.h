@@ -174,6 +174,10 @@ int32_t zc_sweep(void* e, const double H0[9], zc_sweep_info* info, zc_sweep_peak
  * indices, e.g. the learned prior's proposals); best_peak (nullable): the sweep's best raw peak
  * score over the fixed image's area (the learned search's fallback test) */
 int32_t zc_search_cells(void* e, const uint32_t* cells, uint32_t k, zc_search_result* out, double* best_peak);
+/* zc_sweep over k (rotation index, scale index) cells of the grid only (cells: 2k indices; NULL:
+ * the whole grid); best in the cells' order */
+int32_t zc_sweep_cells(void* e, const double H0[9], const uint32_t* cells, uint32_t k, zc_sweep_info* info,
+                       zc_sweep_peak* peaks, size_t cap, float* best, size_t nbest);
 /* the score map of the last sweep batch's first candidate; dims = (nx, ny) */
 int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
 /* candidates per sweep batch (0: 64); process-wide */

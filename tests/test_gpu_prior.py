@@ -159,3 +159,43 @@ def test_training_steps_reduce_the_loss(env):
     for _ in range(40):
         last = float(net.step(x, y, 1e-3))
     assert last < 0.7 * first, (first, last)
+
+
+def test_trained_prior_gains_over_untrained(env):
+    """Held-out 512² pairs (random rotation over the circle, scale 0.8–1.25, shift, remap): the
+    trained network's top-8 cells hold the sweep's argmax cell (up to the spectrum's 180°
+    ambiguity and one rotation step) far more often than an untrained network's, and the search
+    over the grid cells inside its top-8 classes (θ and θ + 180°) reaches the full search's pose."""
+    lib, torch = env
+    import zcmir_amd
+    from zcmir_amd import prior
+
+    net = prior.Net.load()            # the committed weights: fails loudly when missing
+    rnd = prior.Net(seed=123)
+    rng = np.random.default_rng(777)
+    n = 48
+    tex = torch.tensor(np.stack([synth._texture(512, 512, rng) for _ in range(12)]).astype(np.float32), device="cuda")
+    Hs, th, sg, rm = prior.random_poses(rng, n, 512)
+    mv, fx = prior.make_pairs(tex, rng.integers(0, 12, n), Hs, rm)
+    reg = zcmir_amd.Registrar(half=False)
+    lab, _ = prior.sweep_labels(reg, mv, fx, grid=64)
+    reg.close()
+    x = prior.features(mv, fx)
+    y = torch.tensor(lab, device="cuda")
+    trained, untrained = prior.evaluate(net, x, y), prior.evaluate(rnd, x, y)
+    assert trained["top8_near"] >= 0.75 and untrained["top8_near"] <= 0.4, (trained, untrained)
+    assert trained["top8_near"] - untrained["top8_near"] >= 0.4
+    reg = zcmir_amd.Registrar(half=False, sw_nth=72, sw_smin=0.8, sw_smax=1.25, sw_ns=9, sw_grid=128)
+    same = 0
+    for i in range(4):
+        reg.set_images(mv[i].cpu().numpy(), fx[i].cpu().numpy(), preprocess=False)
+        reg.pose = np.eye(3)
+        full = reg.search()
+        reg.pose = np.eye(3)
+        lrn, info = prior.learned_search(reg, net, mv[i], fx[i])
+        assert len(info["fine"]) <= 48
+        pts = np.array([[1, 1, 1], [512, 1, 1], [1, 512, 1], [512, 512, 1]], float).T
+        a, b = lrn.H @ pts, full.H @ pts
+        same += np.abs(a[:2] / a[2] - b[:2] / b[2]).max() <= 0.5
+    reg.close()
+    assert same >= 3, same

@@ -136,6 +136,7 @@ _SIGS = {
     "zc_rank": ([P, U32, f32p, SZ, u32p, u32p, f32p], I32),
     "zc_bench_shift_map": ([P, f64p, U32, U32, U32, f64p], I32),
     "zc_sweep": ([P, f64p, P, P, SZ, f32p, SZ], I32),
+    "zc_sweep_cells": ([P, f64p, u32p, U32, P, P, SZ, f32p, SZ], I32),
     "zc_sweep_map": ([P, f32p, SZ, u32p], I32),
     "zc_sweep_tune": ([U32], None),
     "zc_stack_set": ([P, U32, P, u32p, P, u32p], I32),
@@ -168,6 +169,13 @@ def load():
     p = lib_path()
     if not p.exists():
         raise FileNotFoundError(f"{p} not found: build the HIP library with `python -m zcmir_amd.build`")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7 (the same soname as the
+    # system's), and whichever loads first serves both. PyTorch fails to initialize on a runtime
+    # other than its own, so when it is installed it loads first (the learned prior uses it).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(p))
     for name, (args, res) in _SIGS.items():
         fn = getattr(lib, name)

@@ -88,6 +88,9 @@ zc::SweepOpts sweep_opts(Engine* e, zc::SweepGrid& gv) {
 
 extern "C" {
 
+int32_t zc_sweep_cells(void* p, const double* H0, const uint32_t* cells, uint32_t k, SweepInfo* info,
+                       zc::SweepPeak* peaks, size_t cap, float* best, size_t nbest);
+
 const char* zc_version(void) { return "0.1.4+mi355x"; }
 
 void* zc_create(const char* dev) {
@@ -362,12 +365,22 @@ int32_t zc_bench_shift_map(void* p, const double* H, uint32_t flags, uint32_t ca
 // Writes up to `cap` raw peaks and each candidate's best score; returns the peak count.
 int32_t zc_sweep(void* p, const double* H0, SweepInfo* info, zc::SweepPeak* peaks, size_t cap, float* best,
                  size_t nbest) {
+  return zc_sweep_cells(p, H0, nullptr, 0, info, peaks, cap, best, nbest);
+}
+
+// zc_sweep over k given (rotation index, scale index) cells of the grid only (cells: 2·k; NULL:
+// the whole grid); `best` in the cells' order.
+int32_t zc_sweep_cells(void* p, const double* H0, const uint32_t* cells, uint32_t k, SweepInfo* info,
+                       zc::SweepPeak* peaks, size_t cap, float* best, size_t nbest) {
   Engine* e = static_cast<Engine*>(p);
   int32_t count = 0;
   const int32_t rc = guard(e, "sweep", [&] {
     e->need_images();
     zc::SweepGrid gv;
     zc::SweepOpts o = sweep_opts(e, gv);
+    std::vector<std::array<uint32_t, 2>> cl(cells ? k : 0);
+    for (size_t i = 0; i < cl.size(); ++i) cl[i] = {cells[2 * i], cells[2 * i + 1]};
+    if (cells) o.cells = &cl;
     if (H0) o.H0 = mat(H0);
     const zc::SweepOut r = e->sweep(o);
     if (info) *info = {r.nc, r.n, r.grid, (uint32_t)r.peaks.size(), r.cell, r.D, r.ox, r.oy};

@@ -174,12 +174,85 @@ def features(moving, fixed):
 
 
 def propose(net, moving, fixed, k=8):
-    """The k most likely (rotation index, scale index) cells for each pair, best first."""
+    """The k most likely (rotation, scale) classes of each pair, best first, each as its two cells
+    (θ, θ + 180°): the magnitude spectrum cannot tell them apart, so a class's belief is the
+    log-sum of its two cells' logits and the SMI scores of both cells decide."""
     import torch
 
     _, z = net.forward(features(moving, fixed))
-    top = torch.topk(z, k, dim=1).indices.cpu().numpy()
-    return [[(int(c) // N_SG, int(c) % N_SG) for c in row] for row in top]
+    half = N_TH // 2
+    zc = torch.logsumexp(z.view(-1, 2, half, N_SG), dim=1).view(-1, half * N_SG)
+    top = torch.topk(zc, k, dim=1).indices.cpu().numpy()
+    out = []
+    for row in top:
+        cells = []
+        for c in row:
+            t, s = int(c) // N_SG, int(c) % N_SG
+            cells += [(t, s), (t + half, s)]
+        out.append(cells)
+    return out
+
+
+def fine_cells(cells, n_theta=72):
+    """The (rotation, scale) cells of an n_theta-rotation sweep grid (same 9 scales) inside the
+    proposed 15° cells: each coarse rotation index covers the fine rotations within ±7.5°."""
+    per = n_theta // N_TH
+    out = []
+    for it, is_ in cells:
+        for d in range(-(per // 2), per - per // 2):
+            c = ((per * it + d) % n_theta, is_)
+            if c not in out:
+                out.append(c)
+    return out
+
+
+def check_score(reg, fine):
+    """How well the cells explain the pair: the sweep over them on the 128² canvas, then the
+    forward SMI per overlapping pixel (Φ = S / n, the mean squared whitened correlation, between
+    0 and 4) at the best peak's pose. Unlike S, Φ does not grow with the overlap, so one
+    threshold serves pairs of any size and overlap."""
+    s = reg.settings
+    keep = {"sw_full": s["sw_full"], "sw_grid": s["sw_grid"]}
+    reg.configure(sw_full=False, sw_grid=128)
+    try:
+        _, peaks, _ = reg.sweep(cells=fine)
+    finally:
+        reg.configure(**keep)
+    if len(peaks) == 0:
+        return 0.0
+    H = peaks[int(np.argmax(peaks["score"]))]["H"]
+    n = float(reg.moments(H)[0])
+    return float(reg.score(H).fwd) / n if n > 0 else 0.0
+
+
+def calibrate(reg, net, moving, fixed, poses, thetas, sigmas, q=0.05):
+    """The fallback threshold: the q-quantile of check_score over held-out pairs at the cells that
+    hold their true poses (moving, fixed: CUDA tensors [n, h, w]; poses, thetas, sigmas: their
+    candidate-convention truths)."""
+    vals = []
+    for i in range(len(poses)):
+        reg.set_images(moving[i].cpu().numpy(), fixed[i].cpu().numpy(), preprocess=False)
+        t, s_ = truth_cell(thetas[i], sigmas[i])
+        vals.append(check_score(reg, fine_cells([(t, s_)], int(reg.settings["sw_nth"]))))
+    return float(np.quantile(vals, q))
+
+
+def learned_search(reg, net, moving, fixed, k=8, threshold=0.0):
+    """The search with the prior: the network's top-k classes (2k cells: θ and θ + 180°), the sweep
+    over the grid cells inside them (the registrar's sweep settings: 72 rotations × 9 scales), NMS
+    and Gauss–Newton. When the proposals' check_score falls below 0.8 × threshold (calibrate), the
+    full search runs from the same start instead. moving, fixed: CUDA tensors [h, w] of the
+    registrar's images.
+
+    Returns (Search, info dict: proposed cells, fine cells, check score, fell_back)."""
+    H0 = reg.pose
+    cells = propose(net, moving[None], fixed[None], k)[0]
+    fine = fine_cells(cells, int(reg.settings["sw_nth"]))
+    score = check_score(reg, fine)
+    fell_back = score < 0.8 * threshold
+    reg.pose = H0
+    res = reg.search() if fell_back else reg.search_cells(fine)[0]
+    return res, dict(cells=cells, fine=fine, check_score=score, fell_back=fell_back)
 
 
 # ── training data: pairs the method warps and labels ──────────────────────────────────────
