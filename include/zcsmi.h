@@ -106,7 +106,9 @@ int32_t zc_shift_map(void* e, const double H[9], uint32_t flags, uint32_t cap, z
                      size_t map_len);
 /* exports.zig:442: the settings' score, resolution (map_res) and overlap floor (min_overlap) */
 int32_t zc_shift_map_set(void* e, const double H[9], zc_shift_result* out, float* map, size_t map_len);
-/* exports.zig:449 (returns -1 "Unsupported" in this build) */
+/* exports.zig:449: the roto-scale map at H about c (null: the moving centroid), symmetric: averaged
+ * with the inverse-frame map (engine.zig:836-879); map / fwd / inv nullable, each len >= n*n floats.
+ * Follows the settings exact, calibrated, map_res (grid <= 2048 angles), rs_smin, rs_smax. */
 int32_t zc_rs_map(void* e, const double H[9], const double c[2], uint32_t symmetric, zc_rs_result* out, float* map,
                   float* fwd, float* inv, size_t len);
 /* exports.zig:79: one 2-D FFT of `planes` n x n complex planes in place (interleaved f32;
@@ -122,8 +124,10 @@ void zc_get_pose(void* e, double H[9]);
  * 4-iteration Gauss-Newton screen, 25 iterations on the finalists; the best pose becomes the
  * engine's pose when it beats the current score */
 int32_t zc_search(void* e, zc_search_result* out);
-/* exports.zig:405: Gauss-Newton climb from the engine's pose (g_iters, g_a0, g_decay, g_steps;
- * no shift-map hop or spline in this library); the result becomes the engine's pose */
+/* exports.zig:405 (gclimb.zig:313-489): the climb from the engine's pose: per iteration a
+ * shift-map hop (setting hop), a Gauss-Newton block, and with hop_fm a roto-scale hop and another
+ * block; at most g_iters iterations, two without a move end it; no spline stage in this library.
+ * The result becomes the engine's pose */
 int32_t zc_climb(void* e, zc_climb_result* out);
 /* exports.zig:221 */
 void zc_cancel(void* e);

@@ -417,3 +417,99 @@ def gn_batch(feat_a, feat_b, Hs, iters, group="homography", exact=True, symmetri
         act = [i for i in act if i in moved]
         it += 1
     return [(H, s) for H, s in cur]
+
+
+def climb(prep_a: dict, prep_b: dict, H0, n_iters: int, hop: bool = True, hop_fm: bool = False,
+          group: str = "homography", exact: bool = True, symmetric: bool = True, calibrated: bool = False,
+          cap: int = 1024, min_overlap: float = 0.0, smin: float = 0.2, smax: float = 5.0):
+    """The climb (gclimb.zig:313-489 run; climb.wgsl:229-296 gn_cands, 320-376 hop_cand / rs_cand,
+    415-468 climb_select, 840-858 iter_begin / iter_end), SMI family. Per iteration: a shift-map
+    hop, a Gauss–Newton block of up to 3 steps (5 lengths each; the block ends at a step that beats
+    nothing), and with hop_fm a roto-scale hop then another block. A candidate replaces the pose if
+    it beats its score by max(1e-6 |s|, 1e-12) (the best such). Stop after 2 iterations without a
+    move (1 when no hop is on) or n_iters. The maps' frame and roto-scale centre are fixed for
+    batches of 8 iterations. Returns (H, score, iterations, moved)."""
+    from oracle import smi_ref as O
+
+    fa, fb = prep_a["feat"], prep_b["feat"]
+    _, lh, lw = fa.shape
+    _, rh, rw = fb.shape
+    nk = n_keys(group)
+    Hc = np.asarray(H0, np.float64).reshape(3, 3).copy()
+    g = pair_grad(fa, fb, Hc, exact, group, symmetric, True)
+    cur = g["score"]
+    start = cur
+    state = dict(moved=False, n_rec=0)
+
+    def select(cands):
+        nonlocal Hc, cur, g
+        best, bs = None, cur
+        for Hk in cands:
+            sc = pair_score(fa, fb, Hk, exact, symmetric)
+            if sc > cur + max(1e-6 * abs(cur), 1e-12) and (best is None or sc > bs):
+                best, bs = Hk, sc
+        if best is None:
+            return False
+        Hc, cur = best, bs
+        state["moved"] = True
+        state["n_rec"] += 1
+        g = pair_grad(fa, fb, Hc, exact, group, symmetric, True)
+        return True
+
+    def gn_cands():
+        A = g["hess"].copy()
+        for k in range(nk):
+            A[k, k] += 1e-4 * max(A[k, k], 1e-12)
+        step = solve_linear(A, 0.5 * g["grad"][:nk])
+        if step is None or not np.all(np.abs(step) < 1e30):
+            return []
+        sn = math.sqrt(float(np.sum(step ** 2)))
+        out = []
+        for mu in (0.5, 1.0, 2.0, 4.0, 8.0):
+            nrm = sn * mu
+            sc = mu * (0.25 / nrm if nrm > 0.25 else 1.0)
+            x = np.zeros(8)
+            x[:nk] = sc * step
+            out.append(compose_n(Hc, x, group, rw, rh))
+        return out
+
+    def gn_block():
+        for _ in range(3):
+            if not select(gn_cands()):
+                break
+
+    it = stall = 0
+    done = False
+    while not done:
+        fr = O.reg_frame(Hc, lw, lh, rw, rh)
+        rs_c = O.moving_centroid(Hc, lw, lh)
+        mirror = Hc[0, 0] * Hc[1, 1] - Hc[0, 1] * Hc[1, 0] > 0
+        for _ in range(8):
+            if done:
+                break
+            state["moved"] = False
+            if hop:
+                _, info = O.shift_map(prep_a, prep_b, Hc, exact, calibrated and exact, cap, min_overlap, frame=fr)
+                T = np.array([[1, 0, -info["dx"]], [0, 1, -info["dy"]], [0, 0, 1.0]])
+                select([T @ Hc])
+            if hop or not hop_fm:
+                gn_block()
+            if hop_fm:
+                fwd, p = O.rs_map_dir(prep_a, prep_b, Hc, rs_c, None, None, exact, calibrated and exact, cap, frame=fr)
+                m = O.rs_finish(fwd, None, mirror, p["dlam"], smin, smax)
+                n = p["n"]
+                lag = O.peak_lag(O.argmax_first(m), n, n)
+                dth = -(lag[0] / max(p["n_th"], 1)) * 2.0 * math.pi
+                dsg = -lag[1] * p["dlam"]
+                if not (abs(dth) < 2e-4 and abs(dsg) < 2e-4):
+                    s_ = math.exp(dsg)
+                    a, b, d, e = s_ * math.cos(dth), -s_ * math.sin(dth), s_ * math.sin(dth), s_ * math.cos(dth)
+                    S = np.array([[a, b, rs_c[0] - a * rs_c[0] - b * rs_c[1]],
+                                  [d, e, rs_c[1] - d * rs_c[0] - e * rs_c[1]], [0, 0, 1.0]])
+                    select([project_group(S @ Hc, group)])
+                gn_block()
+            stall = 0 if state["moved"] else stall + 1
+            it += 1
+            if (not state["moved"] and (not (hop or hop_fm) or stall >= 2)) or it >= n_iters:
+                done = True
+    return Hc, cur, it, bool(state["n_rec"] > 0 and cur > start)

@@ -553,16 +553,17 @@ def peak_lag(idx: int, nx: int, ny: int):
 
 def shift_map(prep_a: dict, prep_b: dict, H, exact: bool = True, calibrated: bool = False,
               cap: int = MAP_CAP, min_overlap: float = 0.0, sizes: str = "ladder",
-              plan: tuple | None = None, direct: bool = False):
+              plan: tuple | None = None, direct: bool = False, frame: tuple | None = None):
     """Dense shift map of pose H (smi.zig:540-572 shiftMap/shiftMapFrom; engine.zig:795-819).
 
     cap 0: automatic plan; else linearCorrPlanRect at largest_smooth(cap). `plan` overrides
-    (nx, ny, cw, ch) (full-resolution maps: cw = canvas width). Returns (map, info dict).
+    (nx, ny, cw, ch) (full-resolution maps: cw = canvas width). frame: (ox, oy, w, h) instead of
+    H's own (smi.zig:548 shiftMapAt: the climb's batch frame). Returns (map, info dict).
     """
     sa, sb = prep_a["stack"], prep_b["stack"]
     _, lh, lw = sa.shape
     _, rh, rw = sb.shape
-    ox, oy, W, Hh = reg_frame(H, lw, lh, rw, rh)
+    ox, oy, W, Hh = reg_frame(H, lw, lh, rw, rh) if frame is None else frame
     Hc = canvas_h(H, ox, oy)
     ca = warp_stack(sa, Hc, W, Hh)
     cb = paste_stack(sb, ox, oy, W, Hh)
@@ -596,6 +597,168 @@ def shift_map(prep_a: dict, prep_b: dict, H, exact: bool = True, calibrated: boo
                 corr=corr, floor=max(MIN_N, floor_n), area=area, area_px=area_px,
                 thresholds=([float(max(MIN_N, floor_n)), 10.0 * max(area, 1.0)] if (exact and calibrated)
                             else [max(float(max(MIN_N, floor_n)), 0.08 * max(float(corr[0, 0, 0]), 0.0))]))
+    return m, info
+
+
+# ── roto-scale (log-polar) map (smi.zig:577-651; smi.wgsl:1434-1452, 1708-1731, 778-796) ──
+MAX_FFT = 4096        # the longest FFT line of the HIP build (maps at one cell per pixel)
+
+
+def moving_centroid(H, lw: int, lh: int):
+    """Mean of the moving image's four corners mapped by H (lie.zig:319-329 movingCentroid)."""
+    pts = [map_pt(H, x, y) for x, y in ((1.0, 1.0), (float(lw), 1.0), (float(lw), float(lh)), (1.0, float(lh)))]
+    return (sum(p[0] for p in pts) * 0.25, sum(p[1] for p in pts) * 0.25)
+
+
+def rs_plan(W: int, Hh: int, ox: int, oy: int, c, r0=None, r1=None, cap: int = MAP_CAP,
+            sizes: str = "ladder") -> dict:
+    """Grid of a roto-scale map on a W x Hh canvas about c (fixed pixels, 1-based)
+    (smi.zig:590-620): radii, n = n_theta (FFT width), n_lambda = max(16, n / 2), log spacing."""
+    cx = c[0] - 1 - ox
+    cy = c[1] - 1 - oy
+    if not r1:
+        r1 = max(math.hypot(px - cx, py - cy)
+                 for px, py in ((0, 0), (W - 1, 0), (W - 1, Hh - 1), (0, Hh - 1)))
+    if r0 is None:
+        r0 = max(2.0, min(r1 / 16.0, 8.0))
+    r_out = max(r1, 2.0)
+    r_in = min(max(r0, 1e-3), r_out / 8.0)
+    n_ring = max(64, int(math.ceil(2.0 * math.pi * r_out)))
+    if cap == 0:
+        n = min(max(1 << (n_ring - 1).bit_length(), MAP_AUTO_MIN), MAP_AUTO_MAX)
+    else:
+        n = min(plan_size(n_ring, 1, sizes), largest_smooth(min(cap, MAX_FFT), sizes))
+    n_lam = max(16, n >> 1)
+    log0 = math.log(r_in)
+    span = math.log(r_out) - log0
+    return dict(n=n, n_th=n, n_lam=n_lam, log0=log0, span=span, dlam=span / n_lam, dth=2.0 * math.pi / n,
+                r0=r_in, r1=r_out, cx=cx, cy=cy)
+
+
+def sample_plane_z(plane: np.ndarray, xs: np.ndarray, ys: np.ndarray) -> np.ndarray:
+    """Bilinear sample at pixel coordinates (pixel i at i), 0 unless all four taps are inside
+    (smi.wgsl:1418-1432 sample_plane_z)."""
+    h, w = plane.shape
+    x0 = np.floor(xs).astype(np.int64)
+    y0 = np.floor(ys).astype(np.int64)
+    ok = (x0 >= 0) & (y0 >= 0) & (x0 + 1 < w) & (y0 + 1 < h)
+    xa = np.clip(x0, 0, w - 1)
+    ya = np.clip(y0, 0, h - 1)
+    xb = np.clip(x0 + 1, 0, w - 1)
+    yb = np.clip(y0 + 1, 0, h - 1)
+    fx = xs - x0
+    fy = ys - y0
+    v = (plane[ya, xa] * (1 - fx) + plane[ya, xb] * fx) * (1 - fy) + (plane[yb, xa] * (1 - fx) + plane[yb, xb] * fx) * fy
+    return np.where(ok, v, 0.0)
+
+
+def spatial_logpolar(canvas: np.ndarray, p: dict) -> np.ndarray:
+    """Canvas stack (5, H, W) on the log-polar grid: rows lambda (radius r_in e^((y+1/2) dlam)),
+    columns theta ((x+1/2) 2 pi / n_th) (smi.wgsl:1437-1452 spatial_logpolar)."""
+    y, x = np.mgrid[0:p["n_lam"], 0:p["n_th"]].astype(np.float64)
+    rho = np.exp(p["log0"] + (y + 0.5) / p["n_lam"] * p["span"])
+    th = (x + 0.5) / p["n_th"] * 2.0 * math.pi
+    xs = p["cx"] + rho * np.cos(th)
+    ys = p["cy"] + rho * np.sin(th)
+    return np.stack([sample_plane_z(canvas[k], xs, ys) for k in range(canvas.shape[0])])
+
+
+def rs_map_dir(prep_a: dict, prep_b: dict, H, c, r0=None, r1=None, exact: bool = True,
+               calibrated: bool = False, cap: int = MAP_CAP, direct: bool = False, frame: tuple | None = None):
+    """One direction's roto-scale map (smi.zig:589-634 rsMapFrom): canvas stacks at H, both on the
+    log-polar grid about c, every real plane x rho (pack_pair's area element), circular in theta and
+    linear in log radius on the n x n grid, each correlation x e^(k dlam) dlam dtheta for lag row k
+    (extract_pair), then the per-lag score with area scale 1. Returns (map n x n, plan dict)."""
+    sa, sb = prep_a["stack"], prep_b["stack"]
+    _, lh, lw = sa.shape
+    _, rh, rw = sb.shape
+    ox, oy, W, Hh = reg_frame(H, lw, lh, rw, rh) if frame is None else frame
+    Hc = canvas_h(H, ox, oy)
+    ca = warp_stack(sa, Hc, W, Hh)
+    cb = paste_stack(sb, ox, oy, W, Hh)
+    p = rs_plan(W, Hh, ox, oy, c, r0, r1, cap)
+    n, n_lam = p["n"], p["n_lam"]
+    om = np.exp(p["log0"] + (np.arange(n_lam) + 0.5) / n_lam * p["span"])[:, None]
+    pa = real_planes(spatial_logpolar(ca, p), exact) * om
+    pb = real_planes(spatial_logpolar(cb, p), exact) * om
+    three = prep_a["mode"] == "three"
+    lst = corr_list(exact)
+    keep = kept_corrs(exact, three)
+    pairs = [lst[k] for k in keep]
+    corr_k = (correlations_direct_circ_x if direct else correlations_fft)(pa, pb, n, n, pairs)
+    k = np.arange(n)
+    k = np.where(k > n // 2, k - n, k)
+    scale = np.exp(k * p["dlam"]) * p["dlam"] * p["dth"]
+    corr = np.zeros((len(lst), n, n))
+    corr[keep] = corr_k * scale[None, :, None]
+    area = 1.0
+    area_px = 1.0
+    if exact and calibrated:
+        area_px = corr_area(feature_acf(prep_a["feat"]), feature_acf(prep_b["feat"]))
+        area = max(1.0, area_px)
+    m = combine(corr, exact=exact, calibrated=calibrated, nf=3 if three else 4, min_n=MIN_N, area=area)
+    p.update(canvas=(W, Hh), offset=(ox, oy), corr=corr, area=area)
+    return m, p
+
+
+def correlations_direct_circ_x(ra: np.ndarray, rb: np.ndarray, nx: int, ny: int, pairs) -> np.ndarray:
+    """Brute-force sum_x a(x + t) b(x), circular in x over nx columns (ra, rb: (R, h, nx)) and
+    linear in y (the log-polar map's correlation; the FFT path's independent check)."""
+    _, h, w = ra.shape
+    assert w == nx
+    out = np.zeros((len(pairs), ny, nx))
+    for ty in range(-(h - 1), h):
+        ys, ye = max(0, -ty), min(h, h - ty)
+        for tx in range(nx):
+            for k, (a, b) in enumerate(pairs):
+                out[k, ty % ny, tx] = np.sum(np.roll(ra[a], -tx, axis=1)[ys + ty:ye + ty] * rb[b, ys:ye])
+    return out
+
+
+def rs_finish(fwd: np.ndarray, inv, mirror_theta: bool, dlam: float, smin: float = 0.2, smax: float = 5.0):
+    """Average with the inverse-frame map at the mirrored lag (-y, -x), or (-y, x) for a
+    reflecting pose, then zero the lambda rows whose applied scale exp(-k dlam) lies outside
+    [smin, smax] (smi.zig:636-650 rsFinish; smi.wgsl:1708-1731 rs_finish)."""
+    n = fwd.shape[0]
+    v = fwd.astype(np.float64).copy()
+    if inv is not None:
+        ym = (n - np.arange(n)) % n
+        xm = (n - np.arange(n)) % n if mirror_theta else np.arange(n)
+        v = 0.5 * (v + inv[ym][:, xm])
+    lo = math.log(max(smin if smin > 0 else 0.2, 1e-6))
+    hi = math.log(max(smax if smax > 0 else 5.0, 1.000001))
+    k = np.arange(n)
+    k = np.where(k > n // 2, k - n, k)
+    sg = -k * dlam
+    v[(sg < lo) | (sg > hi), :] = 0.0
+    return v
+
+
+def rs_map(prep_a: dict, prep_b: dict, H, c=None, symmetric: bool = True, exact: bool = True,
+           calibrated: bool = False, cap: int = MAP_CAP, smin: float = 0.2, smax: float = 5.0):
+    """The roto-scale map at H about c (default the moving centroid) (engine.zig:836-879 rsMap):
+    forward map, optional inverse-frame map with the forward radii, finish, argmax. Returns
+    (finished map, info dict with fwd, inv, peak, dth, dsg and the plan)."""
+    H = np.asarray(H, np.float64).reshape(3, 3)
+    _, lh, lw = prep_a["stack"].shape
+    if c is None:
+        c = moving_centroid(H, lw, lh)
+    fwd, p = rs_map_dir(prep_a, prep_b, H, c, None, None, exact, calibrated, cap)
+    inv = None
+    pi = None
+    if symmetric:
+        Hi = np.linalg.inv(H)
+        ci = map_pt(Hi, c[0], c[1])
+        inv, pi = rs_map_dir(prep_b, prep_a, Hi, ci, p["r0"], p["r1"], exact, calibrated, cap)
+        assert pi["n"] == p["n"], "GridMismatch"
+    mirror = H[0, 0] * H[1, 1] - H[0, 1] * H[1, 0] > 0
+    m = rs_finish(fwd, inv, mirror, p["dlam"], smin, smax)
+    n = p["n"]
+    idx = argmax_first(m)
+    lag = peak_lag(idx, n, n)
+    info = dict(p, fwd=fwd, inv=inv, inv_plan=pi, peak=float(m.ravel()[idx]), peak_index=idx,
+                zero=float(m.ravel()[0]), dth=-(lag[0] / max(p["n_th"], 1)) * 2.0 * math.pi, dsg=-lag[1] * p["dlam"],
+                center=tuple(c))
     return m, info
 
 

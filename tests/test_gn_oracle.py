@@ -109,3 +109,28 @@ def test_gn_step_vanishes_at_optimum():
     A = r["hess"] + 1e-4 * np.diag(np.maximum(np.diag(r["hess"]), 1e-12))
     step = G.solve_linear(A, 0.5 * r["grad"])
     assert np.linalg.norm(step) < 5e-3
+
+
+def test_climb_hop_reaches_the_shift_gn_alone_cannot():
+    """The oracle climb (gclimb.zig run): 6.5 px off the truth, beyond the Gauss–Newton basin's
+    reach in 25 iterations of the hop-less climb, the shift-map hop lands on the true shift's cell
+    and the Gauss–Newton block refines it to the same end pose as the climb from 1.5 px off; the
+    Gauss–Newton step there is small (tests/test_api.py:219-221's criterion)."""
+    mv, fx, Ht = synth.shift_pair((96, 96), (5, -3), seed=2)
+    pa, pb = O.prepare(mv), O.prepare(fx)
+    pts = np.array([[1, 1, 1], [96, 1, 1], [1, 96, 1], [96, 96, 1]], float).T
+
+    def err(Ha, Hb):
+        a, b = Ha @ pts, Hb @ pts
+        return np.abs(a[:2] / a[2] - b[:2] / b[2]).max()
+
+    far = Ht @ np.array([[1, 0, 6.5], [0, 1, -4.0], [0, 0, 1]])
+    near = Ht @ np.array([[1, 0, 1.5], [0, 1, -1.0], [0, 0, 1]])
+    Hh, sh, it_h, moved_h = G.climb(pa, pb, far, 25, hop=True)
+    Hn, sn, _, _ = G.climb(pa, pb, near, 25, hop=False)
+    Hg, sg, _, _ = G.climb(pa, pb, far, 25, hop=False)
+    assert moved_h and it_h >= 2 and err(Hh, Hn) < 0.05 and abs(sh - sn) <= 1e-6 * abs(sn)
+    assert err(Hg, Ht) > 1.0 and sg < sh
+    r = G.pair_grad(pa["feat"], pb["feat"], Hh, True, "homography", True, True)
+    A = r["hess"] + 1e-4 * np.diag(np.maximum(np.diag(r["hess"]), 1e-12))
+    assert np.linalg.norm(G.solve_linear(A, 0.5 * r["grad"])) < 5e-3

@@ -40,18 +40,30 @@ def test_gradient_matches_oracle(reg_factory, symmetric, exact, group):
     assert np.abs(g.hess - o["hess"]).max() <= 2e-4 * np.abs(o["hess"]).max()
 
 
-def test_climb_matches_oracle(reg_factory):
+@pytest.mark.parametrize("offset,hop,hop_fm", [((1.5, -1.0, 0.0, 1.0), False, False),
+                                               ((6.5, -4.0, 0.0, 1.0), True, False),
+                                               ((2.0, 1.0, 7.0, 1.06), True, True)])
+def test_climb_matches_oracle(reg_factory, offset, hop, hop_fm):
+    """The climb (shift-map hop, Gauss–Newton blocks, roto-scale hop) from a start pose off the
+    truth by a translation and a turn / rescale about the centre follows the oracle's climb: the
+    same number of iterations and the same end pose within 0.1 px at the corners."""
+    import math
+
     mv, fx, Ht = synth.shift_pair((96, 96), (5, -3), seed=2)
-    H0 = Ht @ np.array([[1, 0, 1.5], [0, 1, -1.0], [0, 0, 1]])
-    reg = reg_factory(half=False, g_iters=25)
+    tx, ty, deg, sc = offset
+    a = math.radians(deg)
+    C = np.array([[1, 0, 48.5], [0, 1, 48.5], [0, 0, 1.0]])
+    R = np.array([[sc * math.cos(a), -sc * math.sin(a), 0], [sc * math.sin(a), sc * math.cos(a), 0], [0, 0, 1]])
+    H0 = C @ R @ np.linalg.inv(C) @ Ht @ np.array([[1, 0, tx], [0, 1, ty], [0, 0, 1]])
+    reg = reg_factory(half=False, g_iters=25, hop=hop, hop_fm=hop_fm)
     reg.set_images(mv, fx, preprocess=False)
     reg.pose = H0
     c = reg.climb()
-    fa, fb = O.prepare(mv)["feat"], O.prepare(fx)["feat"]
-    (Ho, so), = G.gn_batch(fa, fb, [H0], 25, amps=G.geo_amps(0.08, 0.5, 7))
-    assert c.moved and c.iterations >= 1
-    assert _corner_err(c.H, Ht, 96, 96) < 0.5
+    pa, pb = O.prepare(mv), O.prepare(fx)
+    Ho, so, it, moved = G.climb(pa, pb, H0, 25, hop=hop, hop_fm=hop_fm)
+    assert c.moved and moved and c.iterations == it, (c.iterations, it)
     assert _corner_err(c.H, Ho, 96, 96) < 0.1
+    assert _corner_err(c.H, Ht, 96, 96) < 1.5
     assert abs(c.score - so) <= 1e-3 * abs(so)
     assert np.allclose(reg.pose, c.H)
 

@@ -110,6 +110,22 @@ def test_shift_map_warped_resampled(reg_factory):
     assert same
 
 
+@pytest.mark.parametrize("exact,mode", [(True, "laplacian"), (True, "three"), (False, "laplacian")])
+def test_shift_map_warped_full_resolution(reg_factory, exact, mode):
+    """The same warped pose at one cell per canvas pixel: the fixed rectangle sits inside a larger
+    canvas (negative offsets), so the rectangle sums see clipped, shifted rectangles."""
+    mv, fx, _ = _pair((180, 240), (5, 3), seed=7)
+    c = np.array([[1, 0, 120.5], [0, 1, 90.5], [0, 0, 1.0]])
+    t = np.deg2rad(4.0)
+    R = np.array([[1.03 * np.cos(t), -1.03 * np.sin(t), 0], [1.03 * np.sin(t), 1.03 * np.cos(t), 0], [0, 0, 1]])
+    H = c @ R @ np.linalg.inv(c) @ np.array([[1, 0, -6.0], [0, 1, 4.0], [0, 0, 1]])
+    sm, om, info = _map_case(reg_factory, mv, fx, H, exact=exact, mode=mode, cap=1024)
+    assert (sm.cw, sm.ch) == tuple(sm.canvas) and info["offset"][0] < 0 and info["offset"][1] < 0
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same
+
+
 @pytest.mark.parametrize("mode", ["laplacian", "three"])
 def test_shift_map_1024_full_resolution(reg_factory, mode):
     """1024² pair, 8-bit intensities, shift (113, -57), full-resolution 2048² FFT."""

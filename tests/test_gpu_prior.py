@@ -162,7 +162,7 @@ def test_training_steps_reduce_the_loss(env):
 
 
 def test_trained_prior_gains_over_untrained(env):
-    """Held-out 512² pairs (random rotation over the circle, scale 0.8–1.25, shift, remap): the
+    """Held-out pairs at the training size (random rotation over the circle, scale 0.8–1.25, shift, remap): the
     trained network's top-8 cells hold the sweep's argmax cell (up to the spectrum's 180°
     ambiguity and one rotation step) far more often than an untrained network's, and the search
     over the grid cells inside its top-8 classes (θ and θ + 180°) reaches the full search's pose."""
@@ -171,11 +171,12 @@ def test_trained_prior_gains_over_untrained(env):
     from zcmir_amd import prior
 
     net = prior.Net.load()            # the committed weights: fails loudly when missing
+    size = int(net.meta.get("size", 512))
     rnd = prior.Net(seed=123)
     rng = np.random.default_rng(777)
     n = 48
-    tex = torch.tensor(np.stack([synth._texture(512, 512, rng) for _ in range(12)]).astype(np.float32), device="cuda")
-    Hs, th, sg, rm = prior.random_poses(rng, n, 512)
+    tex = torch.tensor(np.stack([synth._texture(size, size, rng) for _ in range(12)]).astype(np.float32), device="cuda")
+    Hs, th, sg, rm = prior.random_poses(rng, n, size)
     mv, fx = prior.make_pairs(tex, rng.integers(0, 12, n), Hs, rm)
     reg = zcmir_amd.Registrar(half=False)
     lab, _ = prior.sweep_labels(reg, mv, fx, grid=64)
@@ -194,7 +195,7 @@ def test_trained_prior_gains_over_untrained(env):
         reg.pose = np.eye(3)
         lrn, info = prior.learned_search(reg, net, mv[i], fx[i])
         assert len(info["fine"]) <= 48
-        pts = np.array([[1, 1, 1], [512, 1, 1], [1, 512, 1], [512, 512, 1]], float).T
+        pts = np.array([[1, 1, 1], [size, 1, 1], [1, size, 1], [size, size, 1]], float).T
         a, b = lrn.H @ pts, full.H @ pts
         same += np.abs(a[:2] / a[2] - b[:2] / b[2]).max() <= 0.5
     reg.close()

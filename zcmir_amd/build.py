@@ -17,7 +17,7 @@ OBJ = HERE / "_build"
 LIB = HERE / "libzcsmi.so"
 ARCH = os.environ.get("ZCSMI_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "cnn.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "abi.cpp"]
+SOURCES = ["kernels.hip", "cnn.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "rs.cpp", "climb.cpp", "abi.cpp"]
 HEADERS = ["fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
            "search.hpp", "lie.hpp"]
 

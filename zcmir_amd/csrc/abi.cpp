@@ -248,6 +248,12 @@ int32_t zc_shift_map_set(void* p, const double* H, zc::ShiftResult* out, float* 
   });
 }
 
+int32_t zc_rs_map(void* p, const double* H, const double* c, uint32_t symmetric, zc::RsResult* out, float* map,
+                  float* fwd, float* inv, size_t len) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "rs_map", [&] { *out = e->rs_map(mat(H), c, symmetric != 0, map, fwd, inv, len); });
+}
+
 void zc_set_pose(void* p, const double* H) {
   Engine* e = static_cast<Engine*>(p);
   if (e) e->H = mat(H);
@@ -416,25 +422,17 @@ int32_t zc_gradient(void* p, const double* H, uint32_t hess, zc::GradResult* out
   });
 }
 
-// exports.zig:405-408: climb from the engine's pose; the result is the engine's new pose. This
-// library's climb is the batched Gauss–Newton refinement (g_iters iterations, the g_a0 / g_decay
-// / g_steps ascent ladder), without the reference's shift-map hops or spline.
+// exports.zig:405-408: climb from the engine's pose (gclimb.zig:313-489): shift-map hops (`hop`),
+// roto-scale hops (`hop_fm`) and Gauss–Newton blocks, g_iters iterations at most; the result is the
+// engine's new pose. No B-spline stage (the spline is not part of this library).
 int32_t zc_climb(void* p, zc::ClimbResult* out) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "climb", [&] {
-    e->need_images();
-    e->cancel = false;
     const auto& s = e->set;
-    const bool homog = s.e("group") == "homography";
-    const auto amps = zc::geo_amps(s.f("g_a0"), s.f("g_decay"), s.u("g_steps") > 0 ? s.u("g_steps") : 7);
-    uint32_t it = 0;
-    const auto r = e->gn_batch(e->own_pair(1), {e->H}, std::max<uint32_t>(1, s.u("g_iters")), homog, s.b("exact"),
-                               s.b("symmetric"), amps, &it)[0];
-    e->H = r.H;
-    for (int k = 0; k < 9; ++k) out->H[k] = r.H[k];
-    out->score = r.score;
-    out->iterations = it;
-    out->moved = r.moved ? 1 : 0;
+    const std::string metric = s.e("metric");
+    if (metric != "smi") throw std::runtime_error("Unsupported metric " + metric + " (smi only)");
+    *out = e->climb(s.u("g_iters"), s.b("hop"), s.b("hop_fm"), s.e("group") == "homography", s.b("exact"),
+                    s.b("symmetric"));
   });
 }
 
@@ -559,7 +557,6 @@ ZC_UNSUPPORTED(zc_canvas, void* e, const double*, uint32_t, void*)
 ZC_UNSUPPORTED(zc_overlay, void* e, const double*, uint32_t, void*, float*, float*, size_t)
 ZC_UNSUPPORTED(zc_tile_heat, void* e, const double*, uint32_t, uint32_t, double*, float*, float*, size_t)
 ZC_UNSUPPORTED(zc_displacement, void* e, float*, size_t)
-ZC_UNSUPPORTED(zc_rs_map, void* e, const double*, const double*, uint32_t, void*, float*, float*, float*, size_t)
 
 int64_t zc_export_warp(void* e, const char*, size_t, void*, size_t) { return unsupported((Engine*)e, "zc_export_warp"); }
 size_t zc_get_ffd(void* e, float*, size_t) {

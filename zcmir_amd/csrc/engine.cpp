@@ -290,15 +290,13 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
 }
 
 MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, bool calib, uint32_t cap,
-                               double min_overlap, bool wait) {
+                               double min_overlap, bool wait, const Frame* frame) {
   if (A.mode != B.mode) throw std::runtime_error("FeatureModeMismatch: set both images with the same smi_features");
   const bool three = A.mode == "three";
-  const int plan = (exact ? 1 : 0) + (three ? 2 : 0);
-  const CorrPlan& pi = kPlans[plan];
   MapOut m{};
   m.exact = exact;
   m.calib = exact && calib;
-  m.fr = reg_frame(Hp, A.w, A.h, B.w, B.h);
+  m.fr = frame ? *frame : reg_frame(Hp, A.w, A.h, B.w, B.h);
   const Mat3 Hc = canvas_h(Hp, m.fr.ox, m.fr.oy);
   const bool ladder = set.e("fft_sizes") == "ladder";
   if (cap == 0) {
@@ -314,12 +312,21 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   if (m.g.nx > (uint32_t)kMaxFft || m.g.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
   const int W = m.fr.w, Hh = m.fr.h;
   const int nx = (int)m.g.nx, ny = (int)m.g.ny, cw = (int)m.g.cw, ch = (int)m.g.ch;
+  // One canvas cell per pixel: the fixed weight plane is the indicator of the pasted rectangle, so
+  // the correlations (a, 0) are rectangle sums (summed-area tables) instead of FFT correlations.
+  const bool rect = cw == W && ch == Hh && W <= 2048;
+  const int plan = (exact ? 1 : 0) + (three ? 2 : 0) + (rect ? 4 : 0);
+  const CorrPlan& pi = kPlans[plan];
   const int64_t N = (int64_t)nx * ny;
   const size_t cn = (size_t)W * Hh;
   canvas_a.ensure(cn * 20);
   canvas_b.ensure(cn * 20);
   spec_a.ensure((size_t)N * 8 * pi.n_cplx);
-  spec_b.ensure((size_t)N * 8 * pi.n_cplx);
+  spec_b.ensure((size_t)N * 8 * pi.b_n_cplx);
+  if (rect) {
+    sat_band.ensure((size_t)pi.n_sat * sat_bands(Hh) * W * 8);
+    sat.ensure((size_t)pi.n_sat * (W + 1) * (Hh + 1) * 8);
+  }
   work.ensure((size_t)N * 8 * pi.n_out);
   corr.ensure((size_t)N * 4 * pi.n_corr);
   map.ensure((size_t)N * 4);
@@ -328,25 +335,33 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   res.ensure(16);
   // canvas stacks
   double Hi[9];
-  const double det = det3(Hc);
-  if (std::fabs(det) >= 1e-12) {
-    const Mat3 inv = {(Hc[4] * Hc[8] - Hc[5] * Hc[7]) / det, (Hc[2] * Hc[7] - Hc[1] * Hc[8]) / det,
-                      (Hc[1] * Hc[5] - Hc[2] * Hc[4]) / det, (Hc[5] * Hc[6] - Hc[3] * Hc[8]) / det,
-                      (Hc[0] * Hc[8] - Hc[2] * Hc[6]) / det, (Hc[2] * Hc[3] - Hc[0] * Hc[5]) / det,
-                      (Hc[3] * Hc[7] - Hc[4] * Hc[6]) / det, (Hc[1] * Hc[6] - Hc[0] * Hc[7]) / det,
-                      (Hc[0] * Hc[4] - Hc[1] * Hc[3]) / det};
-    for (int i = 0; i < 9; ++i) Hi[i] = inv[i];
-  } else {
-    for (int i = 0; i < 9; ++i) Hi[i] = 0;   // every q.z = 0: the warped stack is empty
-  }
+  warp_inverse(Hc, Hi);
   const float2* y_tw = twiddles(ny);
   LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
   LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
   bool ok = true;
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
   LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.b_n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
+  SatView sv{};
+  if (rect) {
+    SatArgs sa{};
+    for (int k = 0; k < pi.n_sat; ++k) sa.plane[k] = pi.sat_plane[k];
+    LAUNCH("sat_build", ok &= launch_sat_build(canvas_a.as<float>(), W, Hh, sa, pi.n_sat, sat_band.as<double>(), sat.as<double>(), st));
+    sv.sat = sat.as<double>();
+    sv.W = W;
+    sv.H = Hh;
+    sv.nx = nx;
+    sv.ny = ny;
+    sv.lx = 0;
+    while ((1 << sv.lx) < nx) ++sv.lx;
+    // the fixed image pasted at (-ox, -oy) (paste_stack)
+    sv.bx0 = std::max(0, -m.fr.ox);
+    sv.bx1 = std::min(W, -m.fr.ox + B.w);
+    sv.by0 = std::max(0, -m.fr.oy);
+    sv.by1 = std::min(Hh, -m.fr.oy + B.h);
+  }
   LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, FftTune::cols));
   LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
@@ -360,7 +375,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   ca.ridge = kRidge;
   const double gcell = (double)W / (double)cw;
   ca.area = std::max(1.0, pair_area(A, B) / (gcell * gcell));
-  LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st));
+  LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st, rect ? &sv : nullptr));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
   ++counts[2];

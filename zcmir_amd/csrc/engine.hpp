@@ -239,11 +239,30 @@ class Engine {
   // map of H into the engine's map buffer; returns its plan and [peak, idx, map[0]]
   // wait=false: enqueue only (the result lands in res_host at the next synchronization)
   MapOut shift_map(const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap, bool wait = true);
+  // frame: the canvas (default: H's own, reg_frame); the climb keeps one frame for a batch of
+  // iterations (gclimb.zig:384-387)
   MapOut shift_map_sides(Side& A, Side& B, const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap,
-                         bool wait = true);
+                         bool wait = true, const Frame* frame = nullptr);
   float* res_host = nullptr;   // pinned
   void fill_shift_result(const MapOut& m, ShiftResult* out) const;
   void read_map(float* out, size_t len, const MapOut& m);
+
+  // Roto-scale map at H about c (null: the moving centroid), smi.zig:577-651, engine.zig:836-879.
+  // Optional outputs (len floats each): the finished map, the forward and inverse-frame maps.
+  RsResult rs_map(const Mat3& H, const double* c, bool symmetric, float* map, float* fwd, float* inv, size_t len);
+  struct RsPlan {
+    int n, n_th, n_lam, px, py;
+    double log0, span, dlam, dth, r0, r1, cx, cy;
+  };
+  // One direction: side A warped by H onto B's canvas, both on the log-polar grid about c
+  // (1-based fixed pixels); r0 < 0 / r1 <= 0: from the canvas. The n x n map into dst.
+  RsPlan rs_map_dir(Side& A, Side& B, const Mat3& H, double c0, double c1, double r0, double r1, bool exact,
+                    bool calib, uint32_t cap, DBuf& dst, const Frame* frame = nullptr);
+  // Finish (symmetric: average with rs_inv) into rs_out and its peak into res_host.
+  void rs_finish(const RsPlan& p, bool symmetric, bool mirror);
+  // The climb from the engine's pose (gclimb.zig:313-489): shift-map hops, roto-scale hops and
+  // Gauss–Newton blocks per iteration; the engine's pose becomes the result.
+  ClimbResult climb(uint32_t n_iters, bool hop, bool hop_fm, bool homography, bool exact, bool symmetric);
 
   void moments(int a_side, int b_side, const Mat3& H, double out[45]);
   // SMI from 45 sums (moments.zig:91-134), FP64
@@ -294,6 +313,8 @@ class Engine {
   }
 
   DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
+  DBuf sat_band, sat;   // rectangle plans: band sums and summed-area tables
+  DBuf rs_fwd, rs_inv, rs_out, rs_tab;   // roto-scale maps: both directions, finished, log-polar tables
   DBuf sw_poses, sw_pose_fixed, sw_zero, sw_ns, sw_part, sw_peaks;
   DBuf gn_jobs, gn_part, gn_sums;
   std::map<int, DBuf> tw;

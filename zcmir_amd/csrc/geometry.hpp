@@ -68,6 +68,22 @@ inline Frame reg_frame(const Mat3& H, int lw, int lh, int rw, int rh) {
 
 inline Mat3 canvas_h(const Mat3& H, int ox, int oy) { return mul3(translate(-(double)ox, -(double)oy), H); }
 
+// The inverse of a canvas pose for the warp kernels; a singular pose (|det| < 1e-12) gives all
+// zeros (every q.z = 0: the warped stack is empty).
+inline void warp_inverse(const Mat3& Hc, double Hi[9]) {
+  const double det = det3(Hc);
+  if (std::fabs(det) >= 1e-12) {
+    const Mat3 inv = {(Hc[4] * Hc[8] - Hc[5] * Hc[7]) / det, (Hc[2] * Hc[7] - Hc[1] * Hc[8]) / det,
+                      (Hc[1] * Hc[5] - Hc[2] * Hc[4]) / det, (Hc[5] * Hc[6] - Hc[3] * Hc[8]) / det,
+                      (Hc[0] * Hc[8] - Hc[2] * Hc[6]) / det, (Hc[2] * Hc[3] - Hc[0] * Hc[5]) / det,
+                      (Hc[3] * Hc[7] - Hc[4] * Hc[6]) / det, (Hc[1] * Hc[6] - Hc[0] * Hc[7]) / det,
+                      (Hc[0] * Hc[4] - Hc[1] * Hc[3]) / det};
+    for (int i = 0; i < 9; ++i) Hi[i] = inv[i];
+  } else {
+    for (int i = 0; i < 9; ++i) Hi[i] = 0;
+  }
+}
+
 // ── FFT length plans ──
 inline bool smooth(uint32_t n, bool ladder) {
   static const uint32_t primes[] = {2, 3, 5, 7, 11, 13};

@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cmath>
 #include <algorithm>
+#include <stdexcept>
 
 #include "fft.hpp"
 #include "fft_regs.hpp"
@@ -360,10 +361,11 @@ struct RowT {
 
 // Resample + real-plane pack + forward row FFT: rows y < ch of the grid; writes n_cplx spectra
 // rows (plane stride L · ny). The stack values of a thread's 16 points are read once.
-template <int L, int PID>
+// TI: the side table (pack_table): which real planes this side transforms.
+template <int L, int TI>
 __global__ __launch_bounds__(RowT<L>::T) void k_pack_rowfft(const float* __restrict__ canvas, Resample r, int ny,
                                                             float2* spec, const float2* __restrict__ tw) {
-  constexpr CorrPlan P = kPlans[PID];
+  constexpr PackTable P = pack_table(TI);
   constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
   __shared__ float2 lds[LINES * STR];
   const int tid = threadIdx.x;
@@ -400,6 +402,91 @@ __global__ __launch_bounds__(RowT<L>::T) void k_pack_rowfft(const float* __restr
       if (y < r.ch) dst[(int64_t)y * L + x] = val;
     });
     __syncthreads();   // the next plane's first exchange reuses the LDS
+  }
+}
+
+// ── roto-scale maps: log-polar pack ───────────────────────────────────────────────────────
+// Cell (x, y) of the log-polar grid samples the canvas stack at c + rho_y (cos th_x, sin th_x)
+// (pixel i at i; 0 unless all four bilinear taps are inside: smi.wgsl:1418-1452), then every real
+// plane is multiplied by rho_y (the area element, smi.wgsl:718-720), and the row FFT follows.
+// Rows y < n_lam hold data; cells x >= n_th are 0 (a padded angle axis).
+__device__ __forceinline__ void logpolar_values(const LogPolarArgs& a, int x, int y, float* s) {
+  const double rho = a.rho[y];
+  const double xs = a.cx + rho * a.cth[x], ys = a.cy + rho * a.sth[x];
+  const double x0d = floor(xs), y0d = floor(ys);
+  const int x0 = (int)x0d, y0 = (int)y0d;
+  if (!(xs >= 0.0 && ys >= 0.0) || x0 + 1 >= a.W || y0 + 1 >= a.H) {
+#pragma unroll
+    for (int p = 0; p < 5; ++p) s[p] = 0.0f;
+    return;
+  }
+  const float fx = (float)(xs - x0d), fy = (float)(ys - y0d);
+  const int64_t n = (int64_t)a.W * a.H, o = (int64_t)y0 * a.W + x0;
+#pragma unroll
+  for (int p = 0; p < 5; ++p) {
+    const float* c = a.canvas + p * n + o;
+    const float v0 = c[0] * (1.0f - fx) + c[1] * fx;
+    const float v1 = c[a.W] * (1.0f - fx) + c[a.W + 1] * fx;
+    s[p] = v0 * (1.0f - fy) + v1 * fy;
+  }
+}
+
+template <int L, int PID>
+__global__ __launch_bounds__(RowT<L>::T) void k_pack_logpolar(LogPolarArgs a, int ny, float2* spec,
+                                                              const float2* __restrict__ tw) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
+  __shared__ float2 lds[LINES * STR];
+  const int tid = threadIdx.x;
+  const int y0 = blockIdx.x * LINES;
+  const int64_t N = (int64_t)L * ny;
+  float s[M][5];
+  float om[M];
+  {
+    float2 dummy[M];
+    int idx = 0;
+    rload<L, M, T, LINES, false>(dummy, tid, [&](int line, int x) {
+      const int y = y0 + line;
+      if (x < a.n_th && y < a.n_lam) {
+        logpolar_values(a, x, y, s[idx]);
+        om[idx] = (float)a.rho[y];
+      } else {
+#pragma unroll
+        for (int p = 0; p < 5; ++p) s[idx][p] = 0.0f;
+        om[idx] = 0.0f;
+      }
+      ++idx;
+      return make_float2(0.0f, 0.0f);
+    });
+  }
+#pragma unroll
+  for (int c = 0; c < P.n_cplx; ++c) {
+    float2 v[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      const float re = real_plane(P.slot_plane[2 * c], s[i]) * om[i];
+      const float im = (2 * c + 1 < P.n_real) ? real_plane(P.slot_plane[2 * c + 1], s[i]) * om[i] : 0.0f;
+      v[i] = make_float2(re, im);
+    }
+    rfft<L, M, T, LINES, false, -1>(v, lds, STR, tw, tid);
+    float2* dst = spec + c * N;
+    rstore<L, M, T, LINES, false>(v, tid, [&](int line, int x, float2 val) {
+      const int y = y0 + line;
+      if (y < a.n_lam) dst[(int64_t)y * L + x] = val;
+    });
+    __syncthreads();
+  }
+}
+
+// rho_y = exp(log0 + (y + 1/2) / n_lam · span), (cos, sin) of th_x = (x + 1/2) / n_th · 2 pi, FP64.
+__global__ __launch_bounds__(256) void k_rs_tables(int n_th, int n_lam, double log0, double span, double* rho,
+                                                   double* cth, double* sth) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n_lam) rho[i] = exp(log0 + ((double)i + 0.5) / (double)n_lam * span);
+  if (i < n_th) {
+    const double th = ((double)i + 0.5) / (double)n_th * 6.283185307179586476925;
+    cth[i] = cos(th);
+    sth[i] = sin(th);
   }
 }
 
@@ -595,7 +682,7 @@ template <int PID>
 __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, const float2* __restrict__ B,
                                               float2* work, int nx, int ny, float scale) {
   constexpr CorrPlan P = kPlans[PID];
-  constexpr int NC = P.n_cplx;
+  constexpr int NC = P.n_cplx, NCB = P.b_n_cplx;
   const int64_t N = (int64_t)nx * ny;
   // blockIdx.y: the batch entry (its NC spectra in A, its n_out outputs in work; B is shared)
   A += (int64_t)blockIdx.y * NC * N;
@@ -608,11 +695,14 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
   if ((ky == 0 || 2 * ky == ny) && kx > nx / 2) return;   // its mirror bin covers it
   const int64_t k = (int64_t)ky * nx + kx, km = (int64_t)my * nx + mx;
   const bool self = k == km;
-  float2 za[NC], zam[NC], zb[NC], zbm[NC];
+  float2 za[NC], zam[NC], zb[NCB], zbm[NCB];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     za[c] = A[c * N + k];
     zam[c] = A[c * N + km];
+  }
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
     zb[c] = B[c * N + k];
     zbm[c] = B[c * N + km];
   }
@@ -810,6 +900,298 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
     }
   }
   // block argmax
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+// ── rectangle sums ────────────────────────────────────────────────────────────────────────
+// When the fixed side's weight plane is the indicator of the rectangle R (the fixed image pasted
+// at full resolution), correlation (a, 0) at lag t is the sum of plane a over R + t clipped to the
+// canvas: four reads of plane a's summed-area table S(y, x) = sum over rows < y, columns < x
+// ((H+1) x (W+1) doubles per plane). The tables are built in three passes: per-band column sums
+// (k_sat_band), their exclusive prefix over bands (k_sat_carry), then every band's rows with a
+// running column carry (k_sat_rows). Canvas planes are read twice; the tables written once.
+constexpr int kSatBand = 16;   // rows per band
+
+__device__ __forceinline__ double real_plane_d(int pl, const float* s) {
+  if (pl < 5) return (double)s[pl];
+  if (s[0] <= 1e-6f) return 0.0;
+  constexpr int TI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+  constexpr int TJ[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+  const int k = pl - 5;
+  return (double)s[1 + TI[k]] * (double)s[1 + TJ[k]] / (double)s[0];
+}
+
+__device__ __forceinline__ double canvas_plane_d(const float* __restrict__ canvas, int64_t n, int64_t at, int pl) {
+  float s[5];
+#pragma unroll
+  for (int p = 0; p < 5; ++p) s[p] = canvas[p * n + at];
+  return real_plane_d(pl, s);
+}
+
+// blockIdx.x band, blockIdx.y table: band[(k · nb + b) · W + x] = column sums of the band's rows.
+__global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canvas, int W, int H, SatArgs sa,
+                                                  double* band) {
+  const int b = blockIdx.x, k = blockIdx.y, nb = gridDim.x, pl = sa.plane[k];
+  const int64_t n = (int64_t)W * H;
+  const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
+  for (int x = threadIdx.x; x < W; x += 256) {
+    double acc = 0.0;
+    for (int y = y0; y < y1; ++y) acc += canvas_plane_d(canvas, n, (int64_t)y * W + x, pl);
+    band[((int64_t)k * nb + b) * W + x] = acc;
+  }
+}
+
+// In place: band sums → exclusive prefix over bands, per (table, column).
+__global__ __launch_bounds__(256) void k_sat_carry(double* band, int W, int nb, int n_tab) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)n_tab * W) return;
+  const int k = (int)(i / W), x = (int)(i - (int64_t)k * W);
+  double* col = band + (int64_t)k * nb * W + x;
+  double acc = 0.0;
+  int b = 0;
+  for (; b + 8 <= nb; b += 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = col[(int64_t)(b + j) * W];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      col[(int64_t)(b + j) * W] = acc;
+      acc += v[j];
+    }
+  }
+  for (; b < nb; ++b) {
+    const double v = col[(int64_t)b * W];
+    col[(int64_t)b * W] = acc;
+    acc += v;
+  }
+}
+
+// Exclusive prefix of v over the 256 threads of the block (lds: 4 doubles).
+__device__ __forceinline__ double block_excl_scan(double v, double* lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  double off = 0.0;
+  for (int j = 0; j < w; ++j) off += lds[j];
+  __syncthreads();   // lds is reused by the next scan
+  return off + inc - v;
+}
+
+// blockIdx.x band, blockIdx.y table; thread t owns columns [t·E, t·E + E), E = ceil(W / 256) <= 8.
+__global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canvas, int W, int H, SatArgs sa,
+                                                  const double* __restrict__ band, double* sat) {
+  constexpr int EMAX = 8;
+  __shared__ double lds[4];
+  const int b = blockIdx.x, k = blockIdx.y, nb = gridDim.x, pl = sa.plane[k];
+  const int E = (W + 255) / 256;
+  const int xs = threadIdx.x * E;
+  const int64_t n = (int64_t)W * H, SW = W + 1;
+  double* S = sat + (int64_t)k * SW * (H + 1);
+  // column carry cc[e] = S(y, xs + e + 1): the row prefix of the earlier bands' column sums
+  double cc[EMAX];
+  {
+    const double* crow = band + ((int64_t)k * nb + b) * W;
+    double tot = 0.0;
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+      cc[e] = (e < E && xs + e < W) ? crow[xs + e] : 0.0;
+      tot += cc[e];
+      cc[e] = tot;
+    }
+    const double off = block_excl_scan(tot, lds);
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) cc[e] += off;
+  }
+  const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
+  if (b == 0) {
+    for (int x = threadIdx.x; x <= W; x += 256) S[x] = 0.0;
+  }
+  for (int y = y0; y < y1; ++y) {
+    double v[EMAX];
+    double tot = 0.0;
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+      v[e] = (e < E && xs + e < W) ? canvas_plane_d(canvas, n, (int64_t)y * W + xs + e, pl) : 0.0;
+      tot += v[e];
+      v[e] = tot;
+    }
+    const double off = block_excl_scan(tot, lds);
+    double* row = S + (int64_t)(y + 1) * SW;
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+      cc[e] += off + v[e];
+      if (e < E && xs + e < W) row[xs + e + 1] = cc[e];
+    }
+    if (threadIdx.x == 0) row[0] = 0.0;
+  }
+}
+
+int sat_bands(int H) { return (H + kSatBand - 1) / kSatBand; }
+
+bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int n_tab, double* band, double* sat,
+                      hipStream_t st) {
+  if (W < 1 || H < 1 || W > 256 * 8 || n_tab < 1 || n_tab > 15) return false;
+  const int nb = sat_bands(H);
+  k_sat_band<<<dim3(nb, n_tab), 256, 0, st>>>(canvas, W, H, sa, band);
+  const int64_t nt = (int64_t)n_tab * W;
+  k_sat_carry<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(band, W, nb, n_tab);
+  k_sat_rows<<<dim3(nb, n_tab), 256, 0, st>>>(canvas, W, H, sa, band, sat);
+  return true;
+}
+
+// Table offsets of the rectangle R + (dx, dy) clipped to the canvas for lag index t.
+struct SatIdx {
+  int64_t o00, o01, o10, o11;   // S(y0, x0), S(y0, x1), S(y1, x0), S(y1, x1)
+  bool empty;
+};
+
+__device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
+  const int px = (int)(t & (v.nx - 1)), py = (int)(t >> v.lx);
+  const int dx = px <= v.nx / 2 ? px : px - v.nx;
+  const int dy = py <= v.ny / 2 ? py : py - v.ny;
+  const int x0 = min(max(v.bx0 + dx, 0), v.W), x1 = min(max(v.bx1 + dx, 0), v.W);
+  const int y0 = min(max(v.by0 + dy, 0), v.H), y1 = min(max(v.by1 + dy, 0), v.H);
+  SatIdx q;
+  const int64_t SW = v.W + 1;
+  q.empty = x1 <= x0 || y1 <= y0;
+  q.o00 = y0 * SW + x0;
+  q.o01 = y0 * SW + x1;
+  q.o10 = y1 * SW + x0;
+  q.o11 = y1 * SW + x1;
+  return q;
+}
+
+__device__ __forceinline__ double sat_sum(const double* __restrict__ S, const SatIdx& q) {
+  if (q.empty) return 0.0;
+  return (S[q.o11] - S[q.o01]) - (S[q.o10] - S[q.o00]);
+}
+
+// k_combine for the rectangle plans: correlations with sat_of >= 0 come from the tables.
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ corr, float* map, int64_t nn,
+                                                      CombineArgs a, SatView sv, float* arg_part) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  const int64_t SP = (int64_t)(sv.W + 1) * (sv.H + 1);
+  SatIdx q = sat_idx(sv, 0);
+  auto C = [&](int num, int64_t t) -> double {
+    if (P.sat_of[num] >= 0) return sat_sum(sv.sat + P.sat_of[num] * SP, q);
+    return (double)corr[(int64_t)P.kept_of[num] * nn + t];
+  };
+  const double n0 = C(0, 0);
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
+    q = sat_idx(sv, t);
+    const float s = (float)lag_score<PID & 3>(C, t, floor_n, a, calib);
+    map[t] = s;
+    if (better(s, (uint32_t)t, bv, bi)) {
+      bv = s;
+      bi = (uint32_t)t;
+    }
+  }
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+// ── roto-scale maps: per-lag score and finish ─────────────────────────────────────────────
+// Lag t of the n x n map: row k (signed, lambda), column kx (theta). The correlations were taken
+// on a px x py grid: px = n (circular in theta) or a padded length >= 2n, whose two linear lags kx
+// and kx - n fold into the circular one; py >= n (linear in lambda). Each correlation is scaled by
+// e^(k dlam) dlam dth (extract_pair's area element, smi.wgsl:778-796).
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine_rs(const float* __restrict__ corr, float* map, RsArgs r,
+                                                    CombineArgs a) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  const int64_t nn = (int64_t)r.px * r.py, total = (int64_t)r.n * r.n;
+  int64_t base = 0, base2 = -1;
+  double sc = 0.0;
+  auto at = [&](int64_t t) {
+    const int ky = (int)(t / r.n), kx = (int)(t - (int64_t)ky * r.n);
+    const int k = ky > r.n / 2 ? ky - r.n : ky;
+    const int row = k < 0 ? k + r.py : k;
+    base = (int64_t)row * r.px + kx;
+    base2 = (r.px != r.n && kx > 0) ? base - r.n + r.px : -1;
+    sc = exp((double)k * r.dlam) * r.dlam * r.dth;
+  };
+  auto C = [&](int num, int64_t) -> double {
+    const float* c = corr + (int64_t)P.kept_of[num] * nn;
+    double v = (double)c[base];
+    if (base2 >= 0) v += (double)c[base2];
+    return v * sc;
+  };
+  at(0);
+  const double n0 = C(0, 0);
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    at(t);
+    map[t] = (float)lag_score<PID>(C, t, floor_n, a, calib);
+  }
+}
+
+// fwd averaged with inv at the mirrored lag (-y, -x), or (-y, x) when !mirror; rows whose applied
+// scale exp(-k dlam) lies outside [e^lo, e^hi] zeroed (smi.wgsl:1708-1731); block argmax partials.
+__global__ __launch_bounds__(256) void k_rs_finish(const float* __restrict__ fwd, const float* __restrict__ inv,
+                                                   int n, int mirror, double dlam, double lo, double hi, float* out,
+                                                   float* arg_part) {
+  const int64_t total = (int64_t)n * n;
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int y = (int)(t / n), x = (int)(t - (int64_t)y * n);
+    float v = fwd[t];
+    if (inv) {
+      const int ym = (n - y) % n, xm = mirror ? (n - x) % n : x;
+      v = 0.5f * (v + inv[(int64_t)ym * n + xm]);
+    }
+    const int k = y > n / 2 ? y - n : y;
+    const double sg = -(double)k * dlam;
+    if (sg < lo || sg > hi) v = 0.0f;
+    out[t] = v;
+    if (better(v, (uint32_t)t, bv, bi)) {
+      bv = v;
+      bi = (uint32_t)t;
+    }
+  }
   __shared__ float sv_[256];
   __shared__ uint32_t si_[256];
   sv_[threadIdx.x] = bv;
@@ -1341,15 +1723,27 @@ void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, floa
 }
 
 template <int L>
-static void pack_rowfft_L(int plan, const float* canvas, const Resample& r, int ny, float2* spec, const float2* tw,
-                          hipStream_t st) {
+static void pack_rowfft_L(int plan, int side, const float* canvas, const Resample& r, int ny, float2* spec,
+                          const float2* tw, hipStream_t st) {
   dim3 g((r.ch + RowT<L>::LINES - 1) / RowT<L>::LINES);
-  switch (plan) {
-    case 0: k_pack_rowfft<L, 0><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
-    case 1: k_pack_rowfft<L, 1><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
-    case 2: k_pack_rowfft<L, 2><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
-    default: k_pack_rowfft<L, 3><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw); break;
+  // plans without rectangle sums share one table for both sides
+  const int ti = plan < 4 ? 2 * plan : 2 * plan + side;
+#define K(TAB) k_pack_rowfft<L, TAB><<<g, RowT<L>::T, 0, st>>>(canvas, r, ny, spec, tw)
+  switch (ti) {
+    case 0: K(0); break;
+    case 2: K(2); break;
+    case 4: K(4); break;
+    case 6: K(6); break;
+    case 8: K(8); break;
+    case 9: K(9); break;
+    case 10: K(10); break;
+    case 11: K(11); break;
+    case 12: K(12); break;
+    case 13: K(13); break;
+    case 14: K(14); break;
+    default: K(15); break;
   }
+#undef K
 }
 
 #define ZC_POW2_SWITCH(L, CALL) \
@@ -1365,8 +1759,8 @@ static void pack_rowfft_L(int plan, const float* canvas, const Resample& r, int 
     default: return false;      \
   }
 
-bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int ch, int nx, int ny, float2* spec,
-                        const float2* tw_x, hipStream_t st) {
+bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
+                        float2* spec, const float2* tw_x, hipStream_t st) {
   Resample r;
   r.W = W;
   r.H = H;
@@ -1377,10 +1771,52 @@ bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int
   r.kx = (int)fminf(fmaxf(ceilf(r.fx - 1e-4f), 1.0f), 8.0f);
   r.ky = (int)fminf(fmaxf(ceilf(r.fy - 1e-4f), 1.0f), 8.0f);
   r.identity = (cw == W && ch == H) ? 1 : 0;
-#define CALL(LL) pack_rowfft_L<LL>(plan, canvas, r, ny, spec, tw_x, st)
+#define CALL(LL) pack_rowfft_L<LL>(plan, side, canvas, r, ny, spec, tw_x, st)
   ZC_POW2_SWITCH(nx, CALL)
 #undef CALL
   return true;
+}
+
+template <int L>
+static void pack_logpolar_L(int plan, const LogPolarArgs& a, int ny, float2* spec, const float2* tw, hipStream_t st) {
+  dim3 g((a.n_lam + RowT<L>::LINES - 1) / RowT<L>::LINES);
+  switch (plan) {
+    case 0: k_pack_logpolar<L, 0><<<g, RowT<L>::T, 0, st>>>(a, ny, spec, tw); break;
+    case 1: k_pack_logpolar<L, 1><<<g, RowT<L>::T, 0, st>>>(a, ny, spec, tw); break;
+    case 2: k_pack_logpolar<L, 2><<<g, RowT<L>::T, 0, st>>>(a, ny, spec, tw); break;
+    default: k_pack_logpolar<L, 3><<<g, RowT<L>::T, 0, st>>>(a, ny, spec, tw); break;
+  }
+}
+
+bool launch_pack_logpolar(int plan, const LogPolarArgs& a, int nx, int ny, float2* spec, const float2* tw_x,
+                          hipStream_t st) {
+  if (plan < 0 || plan > 3 || a.n_th > nx || a.n_lam > ny) return false;
+#define CALL(LL) pack_logpolar_L<LL>(plan, a, ny, spec, tw_x, st)
+  ZC_POW2_SWITCH(nx, CALL)
+#undef CALL
+  return true;
+}
+
+void launch_rs_tables(int n_th, int n_lam, double log0, double span, double* rho, double* cth, double* sth,
+                      hipStream_t st) {
+  const int m = n_th > n_lam ? n_th : n_lam;
+  k_rs_tables<<<(m + 255) / 256, 256, 0, st>>>(n_th, n_lam, log0, span, rho, cth, sth);
+}
+
+void launch_combine_rs(int plan, const float* corr, float* map, const RsArgs& r, const CombineArgs& a,
+                       hipStream_t st) {
+  const int g = combine_blocks((int64_t)r.n * r.n);
+  switch (plan) {
+    case 0: k_combine_rs<0><<<g, 256, 0, st>>>(corr, map, r, a); break;
+    case 1: k_combine_rs<1><<<g, 256, 0, st>>>(corr, map, r, a); break;
+    case 2: k_combine_rs<2><<<g, 256, 0, st>>>(corr, map, r, a); break;
+    default: k_combine_rs<3><<<g, 256, 0, st>>>(corr, map, r, a); break;
+  }
+}
+
+void launch_rs_finish(const float* fwd, const float* inv, int n, bool mirror, double dlam, double lo, double hi,
+                      float* out, float* arg_part, hipStream_t st) {
+  k_rs_finish<<<combine_blocks((int64_t)n * n), 256, 0, st>>>(fwd, inv, n, mirror ? 1 : 0, dlam, lo, hi, out, arg_part);
 }
 
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
@@ -1402,7 +1838,11 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
     case 0: k_cmul<0><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
     case 1: k_cmul<1><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
     case 2: k_cmul<2><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    default: k_cmul<3><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 3: k_cmul<3><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 4: k_cmul<4><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 5: k_cmul<5><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 6: k_cmul<6><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    default: k_cmul<7><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
   }
 }
 
@@ -1413,7 +1853,11 @@ static void rowifft_L(int plan, const float2* work, float* corr, int ny, const f
     case 0: k_rowifft_extract<L, 0><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
     case 1: k_rowifft_extract<L, 1><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
     case 2: k_rowifft_extract<L, 2><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    default: k_rowifft_extract<L, 3><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 3: k_rowifft_extract<L, 3><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 4: k_rowifft_extract<L, 4><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 5: k_rowifft_extract<L, 5><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    case 6: k_rowifft_extract<L, 6><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+    default: k_rowifft_extract<L, 7><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
   }
 }
 
@@ -1428,13 +1872,18 @@ bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, i
 int combine_blocks(int64_t nn) { return (int)std::min<int64_t>((nn + 255) / 256, 4096); }
 
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
-                    hipStream_t st) {
+                    hipStream_t st, const SatView* sv) {
   const int g = combine_blocks(nn);
+  if (plan >= 4 && !sv) throw std::runtime_error("rectangle plan without tables");
   switch (plan) {
     case 0: k_combine<0><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 1: k_combine<1><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 2: k_combine<2><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
-    default: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 3: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 4: k_combine_rect<4><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    case 5: k_combine_rect<5><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    case 6: k_combine_rect<6><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    default: k_combine_rect<7><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
   }
 }
 

@@ -40,7 +40,7 @@ void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, floa
                         hipStream_t st);
 // grid: nx × ny FFT, cw × ch cells resampled from a W × H canvas; writes n_cplx planes of
 // ch rows × nx into spec (plane stride nx·ny). Returns false for an unsupported length.
-bool launch_pack_rowfft(int plan, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
+bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
                         float2* spec, const float2* tw_x, hipStream_t st);
 // column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
 // lines: columns per workgroup (0: the default for the length)
@@ -58,9 +58,49 @@ struct CombineArgs {
   double area, ridge;
 };
 int combine_blocks(int64_t nn);
+// Summed-area tables of the rectangle-sum planes (rectangle plans): plane[k] for table k.
+struct SatArgs {
+  int plane[15];
+};
+int sat_bands(int H);
+// canvas: 5 planes W x H; band: n_tab · sat_bands(H) · W doubles; sat: n_tab · (W+1)(H+1) doubles
+bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int n_tab, double* band, double* sat,
+                      hipStream_t st);
+// The tables and the fixed rectangle [bx0, bx1) x [by0, by1) in canvas cells; nx = 1 << lx.
+struct SatView {
+  const double* sat;
+  int W, H, nx, ny, lx, bx0, bx1, by0, by1;
+};
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
-                    hipStream_t st);
+                    hipStream_t st, const SatView* sv = nullptr);
 void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st);
+
+// ── roto-scale maps ──
+// The canvas stack (5 planes W x H) on the n_th x n_lam log-polar grid about (cx, cy) (canvas
+// pixels): rho, cth, sth tables of launch_rs_tables.
+struct LogPolarArgs {
+  const float* canvas;
+  int W, H, n_th, n_lam;
+  double cx, cy;
+  const double* rho;
+  const double* cth;
+  const double* sth;
+};
+void launch_rs_tables(int n_th, int n_lam, double log0, double span, double* rho, double* cth, double* sth,
+                      hipStream_t st);
+// plans 0-3; nx >= n_th, ny >= n_lam powers of two; writes rows < n_lam of n_cplx spectra
+bool launch_pack_logpolar(int plan, const LogPolarArgs& a, int nx, int ny, float2* spec, const float2* tw_x,
+                          hipStream_t st);
+// The n x n map from correlations on a px x py grid (px = n, or a power of two >= 2n folded).
+struct RsArgs {
+  int n, px, py;
+  double dlam, dth;
+};
+void launch_combine_rs(int plan, const float* corr, float* map, const RsArgs& r, const CombineArgs& a,
+                       hipStream_t st);
+// arg_part: combine_blocks(n·n) pairs; then launch_argmax_finish over them
+void launch_rs_finish(const float* fwd, const float* inv, int n, bool mirror, double dlam, double lo, double hi,
+                      float* out, float* arg_part, hipStream_t st);
 
 // plain 2-D FFT of `planes` n × n planes (tests, benchmarks)
 bool launch_fft2(float2* data, int planes, int n, bool inverse, const float2* tw, hipStream_t st, int lines = 0);

@@ -7,23 +7,35 @@
 // planes move up into slots, correlations keep their numbers. Two real planes share a complex
 // FFT input (slot 2p in re, 2p+1 in im) and two correlations share a complex inverse FFT
 // (the first in re, the second in im).
+//
+// Rectangle plans (`rect`): when the fixed side's weight plane is exactly the indicator of a
+// rectangle in cell space (the fixed image pasted, cells = canvas pixels), every correlation
+// (a, 0) of a moving plane with that weight is the sum of plane a over the rectangle shifted by
+// the lag, read from a summed-area table of plane a in O(1) per lag. Those correlations leave the
+// FFT path; the moving side then transforms only the planes of the remaining correlations (w and
+// the features) and the fixed side only its planes other than w.
 #pragma once
 
 namespace zc {
 
 struct CorrPlan {
-  int n_real;        // real planes (slots)
-  int n_cplx;        // complex spectra per side
-  int n_corr;        // correlations kept
-  int n_out;         // complex inverse FFTs
-  int slot_plane[15];
-  int plane_slot[15];       // -1: left out
-  int ca[45], cb[45];       // kept correlation k: slot of A, slot of B
-  int cnum[45];             // kept correlation k: its number (0..44)
-  int kept_of[45];          // number → kept index, -1 if left out
+  int n_real;               // side A: real planes (slots) transformed
+  int n_cplx;               // side A: complex spectra
+  int n_corr;               // correlations through the FFT
+  int n_out;                // complex inverse FFTs
+  int slot_plane[15];       // side A: slot → plane
+  int plane_slot[15];       // side A: plane → slot, -1 not transformed
+  int b_n_real, b_n_cplx;   // side B
+  int b_slot_plane[15], b_plane_slot[15];
+  int ca[45], cb[45];       // FFT correlation k: slot of A, slot of B
+  int cnum[45];             // FFT correlation k: its number (0..44)
+  int kept_of[45];          // number → FFT correlation index, -1 if not through the FFT
+  int n_sat;                // rectangle-sum planes of side A
+  int sat_plane[15];        // rectangle-sum index → plane of A
+  int sat_of[45];           // number → rectangle-sum index, -1 if not one
 };
 
-constexpr CorrPlan make_plan(bool exact, bool three) {
+constexpr CorrPlan make_plan(bool exact, bool three, bool rect) {
   CorrPlan p{};
   int la[45] = {}, lb[45] = {};
   int len = 0;
@@ -37,24 +49,53 @@ constexpr CorrPlan make_plan(bool exact, bool three) {
     for (int k = 0; k < 10; ++k) { la[len] = 0; lb[len] = 5 + k; ++len; }
   }
   const int n_planes = exact ? 15 : 5;
-  int n_real = 0;
-  for (int pl = 0; pl < 15; ++pl) p.plane_slot[pl] = -1;
+  bool valid[15] = {};
   for (int pl = 0; pl < n_planes; ++pl) {
     const bool fourth = pl == 4 || pl == 8 || pl == 11 || pl == 13 || pl == 14;
-    if (three && fourth) continue;
-    p.plane_slot[pl] = n_real;
-    p.slot_plane[n_real] = pl;
-    ++n_real;
+    valid[pl] = !(three && fourth);
   }
-  p.n_real = n_real;
-  p.n_cplx = (n_real + 1) / 2;
-  for (int c = 0; c < 45; ++c) p.kept_of[c] = -1;
+  // which correlations go where, and which planes each side transforms
+  bool use_a[15] = {}, use_b[15] = {};
+  for (int c = 0; c < 45; ++c) {
+    p.kept_of[c] = -1;
+    p.sat_of[c] = -1;
+  }
+  int sat_idx[15];
+  for (int pl = 0; pl < 15; ++pl) sat_idx[pl] = -1;
+  for (int c = 0; c < len; ++c) {
+    if (!valid[la[c]] || !valid[lb[c]]) continue;
+    if (rect && lb[c] == 0) {
+      if (sat_idx[la[c]] < 0) {
+        sat_idx[la[c]] = p.n_sat;
+        p.sat_plane[p.n_sat++] = la[c];
+      }
+      p.sat_of[c] = sat_idx[la[c]];
+    } else {
+      use_a[la[c]] = true;
+      use_b[lb[c]] = true;
+    }
+  }
+  for (int pl = 0; pl < 15; ++pl) {
+    p.plane_slot[pl] = -1;
+    p.b_plane_slot[pl] = -1;
+  }
+  for (int pl = 0; pl < n_planes; ++pl) {
+    if (use_a[pl]) {
+      p.plane_slot[pl] = p.n_real;
+      p.slot_plane[p.n_real++] = pl;
+    }
+    if (use_b[pl]) {
+      p.b_plane_slot[pl] = p.b_n_real;
+      p.b_slot_plane[p.b_n_real++] = pl;
+    }
+  }
+  p.n_cplx = (p.n_real + 1) / 2;
+  p.b_n_cplx = (p.b_n_real + 1) / 2;
   int kept = 0;
   for (int c = 0; c < len; ++c) {
-    const int sa = p.plane_slot[la[c]], sb = p.plane_slot[lb[c]];
-    if (sa < 0 || sb < 0) continue;
-    p.ca[kept] = sa;
-    p.cb[kept] = sb;
+    if (!valid[la[c]] || !valid[lb[c]] || p.sat_of[c] >= 0) continue;
+    p.ca[kept] = p.plane_slot[la[c]];
+    p.cb[kept] = p.b_plane_slot[lb[c]];
     p.cnum[kept] = c;
     p.kept_of[c] = kept;
     ++kept;
@@ -64,13 +105,36 @@ constexpr CorrPlan make_plan(bool exact, bool three) {
   return p;
 }
 
-// plan id: exact + 2 · three
-constexpr CorrPlan kPlans[4] = {make_plan(false, false), make_plan(true, false), make_plan(false, true),
-                                make_plan(true, true)};
+// plan id: exact + 2 · three + 4 · rect
+constexpr CorrPlan kPlans[8] = {make_plan(false, false, false), make_plan(true, false, false),
+                                make_plan(false, true, false),  make_plan(true, true, false),
+                                make_plan(false, false, true),  make_plan(true, false, true),
+                                make_plan(false, true, true),   make_plan(true, true, true)};
 
-template <int ID>
-struct PlanT {
-  static constexpr CorrPlan p = kPlans[ID];
+// Without rectangle sums both sides transform the same planes.
+static_assert(kPlans[1].n_real == 15 && kPlans[1].b_n_real == 15 && kPlans[1].n_corr == 45, "exact plan");
+static_assert(kPlans[3].n_real == 10 && kPlans[3].n_corr == 28, "three-feature plan");
+static_assert(kPlans[5].n_sat == 15 && kPlans[5].n_real == 5 && kPlans[5].b_n_real == 14 && kPlans[5].n_corr == 30,
+              "exact rectangle plan");
+static_assert(kPlans[7].n_sat == 10 && kPlans[7].n_real == 4 && kPlans[7].b_n_real == 9 && kPlans[7].n_corr == 18,
+              "three-feature rectangle plan");
+static_assert(kPlans[4].n_sat == 5 && kPlans[4].n_corr == 20, "global rectangle plan");
+
+// The side tables the pack kernels use: index 2 · plan + side (side B of a plan without
+// rectangle sums shares side A's table).
+struct PackTable {
+  int n_real, n_cplx;
+  int slot_plane[15];
 };
+
+constexpr PackTable pack_table(int ti) {
+  const CorrPlan& p = kPlans[ti / 2];
+  PackTable t{};
+  const bool b = (ti & 1) != 0;
+  t.n_real = b ? p.b_n_real : p.n_real;
+  t.n_cplx = b ? p.b_n_cplx : p.n_cplx;
+  for (int s = 0; s < 15; ++s) t.slot_plane[s] = b ? p.b_slot_plane[s] : p.slot_plane[s];
+  return t;
+}
 
 }  // namespace zc

@@ -79,6 +79,25 @@ inline std::array<double, 2> map_pt(const Mat3& H, double x, double y) {
   return {X / z, Y / z};
 }
 
+// lie.zig:319-329 movingCentroid: the mean of the moving image's corners under H
+inline std::array<double, 2> moving_centroid(const Mat3& H, double lw, double lh) {
+  const double cs[4][2] = {{1, 1}, {lw, 1}, {lw, lh}, {1, lh}};
+  std::array<double, 2> c{0.0, 0.0};
+  for (const auto& q : cs) {
+    const auto p = map_pt(H, q[0], q[1]);
+    c[0] += 0.25 * p[0];
+    c[1] += 0.25 * p[1];
+  }
+  return c;
+}
+
+// lie.zig:332-341 simAbout: X ↦ c + e^sg R_th (X − c)
+inline Mat3 sim_about(const std::array<double, 2>& c, double th, double sg) {
+  const double s = std::exp(sg);
+  const double a = s * std::cos(th), b = -s * std::sin(th), d = s * std::sin(th), e = s * std::cos(th);
+  return {a, b, c[0] - a * c[0] - b * c[1], d, e, c[1] - d * c[0] - e * c[1], 0, 0, 1};
+}
+
 // One raw peak of the sweep (search.zig:145): score (n in fixed pixels), candidate and grid
 // indices, the candidate's θ and σ, the canvas shift in cells and the corrected pose.
 struct SweepPeak {
