@@ -326,6 +326,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "event_ms_per_map": round(ev_ms, 4),
+        "fft_column_tiling": {str(L): reg.fft_tiling(L) for L in sorted({sm.n, sm.ny})},
         "alg_bytes_per_map": total_alg,
         "model_bytes_per_map": model_bytes,
         "model_frac": round(model_bytes / (wall / a.steps) / 1e9 / HBM_PEAK_GBS, 4),

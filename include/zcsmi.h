@@ -146,6 +146,10 @@ void zc_result_sizes(uint32_t out[4]);
 /* exports.zig:95: lines per workgroup of the batched FFTs for rows / columns (0: default), for
  * FFTs run from now on; process-wide */
 void zc_fft_tune(uint32_t rows, uint32_t cols);
+/* this library's: the column-FFT tiling used for column length L (zc_fft_tune's value, else the
+ * autotuned one when the setting fft_autotune is on, timing the tilings now if not yet done,
+ * else 0 = built-in default); no counterpart in the reference */
+int32_t zc_fft_tiling(void* e, uint32_t L, uint32_t* cols);
 
 /* ── this library's own hooks (no counterpart in the reference) ── */
 /* the rank transform of a side: rank (w*h f32, nullable), hist[257], cdf[258], stats[2] =

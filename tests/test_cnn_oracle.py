@@ -79,5 +79,6 @@ def test_fine_cells_cover_the_coarse_rotation():
 
     assert prior.fine_cells([(0, 4)]) == [(71, 4), (0, 4), (1, 4)]          # −180° ± 5° wraps
     assert prior.fine_cells([(5, 2), (5, 2), (6, 2)]) == [(14, 2), (15, 2), (16, 2), (17, 2), (18, 2), (19, 2)]
+    assert prior.fine_cells([(0, 4)], margin=1) == [(70, 4), (71, 4), (0, 4), (1, 4), (2, 4)]
     assert prior.truth_cell(-180.0, 1.0) == (0, 4) and prior.truth_cell(172.6, 0.8) == (0, 0)
     assert prior.truth_cell(37.0, 1.12) == (14, 6)

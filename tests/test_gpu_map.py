@@ -160,8 +160,8 @@ def test_errors_fail_loudly(reg_factory):
         reg.shift_map(np.eye(3), exact=True, calibrated=False)
     with pytest.raises(RuntimeError, match="BadImage"):
         reg.set_moving(np.zeros((4, 4), np.float32), preprocess=False)
-    with pytest.raises(RuntimeError, match="Unsupported"):
-        reg.set_moving(np.zeros((16, 16), np.float32), preprocess=True)
+    with pytest.raises(RuntimeError, match="FeatureModeMismatch|NoImages|BadImage"):
+        reg.set_moving(np.zeros((6, 16), np.float32), preprocess=True)
     with pytest.raises(RuntimeError, match="UnknownSetting"):
         reg.configure(nope=1)
 
@@ -190,3 +190,17 @@ def test_fft2_every_column_tiling(reg_factory, n):
     finally:
         lib.zc_fft_tune(0, 0)
     assert ran >= 2
+
+
+def test_autotuned_tiling_leaves_maps_unchanged(reg_factory):
+    """The column-FFT autotuner picks among tilings that compute every column with the same
+    stages: the map with the tuned tiling equals the built-in default's bit for bit."""
+    mv, fx, _ = _pair((256, 256), (17, -9))
+    maps = []
+    for tune in (True, False):
+        reg = reg_factory(half=False, fft_autotune=tune)
+        reg.set_images(mv, fx, preprocess=False)
+        if tune:
+            assert reg.fft_tiling(512) in (0, 1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80)
+        maps.append(reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=1024).map)
+    assert np.array_equal(maps[0].view(np.uint32), maps[1].view(np.uint32))

@@ -122,6 +122,7 @@ _SIGS = {
     "zc_ffd_gradient": ([P, f64p, f64p, f64p, SZ], I32),
     "zc_shift_map": ([P, f64p, U32, U32, ctypes.POINTER(ShiftResult), f32p, SZ], I32),
     "zc_shift_map_set": ([P, f64p, ctypes.POINTER(ShiftResult), f32p, SZ], I32),
+    "zc_fft_tiling": ([P, U32, u32p], I32),
     "zc_rs_map": ([P, f64p, f64p, U32, ctypes.POINTER(RsResult), f32p, f32p, f32p, SZ], I32),
     "zc_canvas": ([P, f64p, U32, ctypes.POINTER(Canvas)], I32),
     "zc_overlay": ([P, f64p, U32, ctypes.POINTER(Canvas), f32p, f32p, SZ], I32),

@@ -17,9 +17,9 @@ OBJ = HERE / "_build"
 LIB = HERE / "libzcsmi.so"
 ARCH = os.environ.get("ZCSMI_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "cnn.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "rs.cpp", "climb.cpp", "abi.cpp"]
+SOURCES = ["kernels.hip", "cnn.hip", "pre.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "rs.cpp", "climb.cpp", "abi.cpp"]
 HEADERS = ["fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
-           "search.hpp", "lie.hpp"]
+           "search.hpp", "lie.hpp", "pre.hpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -27,6 +27,11 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast", "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
+
+
+# the preprocessing kernels round every float32 operation as the reference writes it (no fused
+# multiply-add), so that they are reproducible bit for bit
+PER_SOURCE = {"pre.hip": ["-ffp-contract=off"]}
 
 
 def _hipcc() -> str:
@@ -49,7 +54,7 @@ def build(force: bool = False, verbose: bool = False) -> Path:
         if not force and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, hdr_mtime):
             continue
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [hipcc, *FLAGS, *lang, "-c", str(s), "-o", str(o)]
+        cmd = [hipcc, *FLAGS, *PER_SOURCE.get(src, []), *lang, "-c", str(s), "-o", str(o)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

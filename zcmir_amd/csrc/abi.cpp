@@ -156,13 +156,7 @@ int32_t zc_set_image_device(void* p, uint32_t side, const float* data, uint32_t 
 
 int32_t zc_set_image_rgba(void* p, uint32_t side, const uint8_t* rgba, uint32_t w, uint32_t h) {
   Engine* e = static_cast<Engine*>(p);
-  return guard(e, "set_image_rgba", [&] {
-    // RGBA images are always preprocessed (exports.zig:60-63): allowed only with preprocessing off
-    std::vector<float> g((size_t)w * h);
-    for (size_t i = 0; i < g.size(); ++i)
-      g[i] = (0.299f * rgba[4 * i] + 0.587f * rgba[4 * i + 1] + 0.114f * rgba[4 * i + 2]) / 255.0f;
-    e->set_image((int)side, g.data(), (int)w, (int)h, true);
-  });
+  return guard(e, "set_image_rgba", [&] { e->set_image_rgba((int)side, rgba, (int)w, (int)h); });
 }
 
 void zc_image_size(void* p, uint32_t side, uint32_t out[2]) {
@@ -179,6 +173,7 @@ int32_t zc_work_image(void* p, uint32_t side, float* out, size_t len) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "work_image", [&] {
     if (side > 1 || !e->sides[side].has) throw std::runtime_error("NoImage");
+    if (e->sides[side].baked != e->bake_key(e->sides[side])) e->bake(e->sides[side]);
     const auto& s = e->sides[side];
     const size_t n = (size_t)s.w * s.h;
     if (len < n) throw std::runtime_error("BufferTooSmall");
@@ -191,6 +186,7 @@ int32_t zc_features(void* p, uint32_t side, float* out, size_t len) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "features", [&] {
     if (side > 1 || !e->sides[side].has) throw std::runtime_error("NoImage");
+    if (e->sides[side].baked != e->bake_key(e->sides[side])) e->bake(e->sides[side]);
     const auto& s = e->sides[side];
     const size_t n = 4 * (size_t)s.w * s.h;
     if (len < n) throw std::runtime_error("BufferTooSmall");
@@ -327,6 +323,7 @@ int32_t zc_rank(void* p, uint32_t side, float* rank, size_t len, uint32_t* hist,
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "rank", [&] {
     if (side > 1 || !e->sides[side].has) throw std::runtime_error("NoImage");
+    if (e->sides[side].baked != e->bake_key(e->sides[side])) e->bake(e->sides[side]);
     const auto& s = e->sides[side];
     const size_t n = (size_t)s.w * s.h;
     if (rank) {
@@ -575,5 +572,14 @@ void zc_fft_tune(uint32_t rows, uint32_t cols) {
   zc::FftTune::cols = (int)cols;
 }
 void zc_fft_tune_wg(uint32_t, uint32_t, uint32_t) {}
+
+// The column-FFT tiling the engine uses for column length L (autotuning it now if needed).
+int32_t zc_fft_tiling(void* p, uint32_t L, uint32_t* cols) {
+  Engine* e = static_cast<Engine*>(p);
+  return guard(e, "fft_tiling", [&] {
+    if (L < 32 || L > 4096 || (L & (L - 1))) throw std::runtime_error("FftLength: a power of two, 32..4096");
+    *cols = (uint32_t)e->col_lines((int)L);
+  });
+}
 
 }  // extern "C"

@@ -4,10 +4,12 @@
 // reason kept for zc_error (exports.zig:1-16). No C++ exception crosses the ABI.
 #include "engine.hpp"
 #include "plan.hpp"
+#include "pre.hpp"
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <sstream>
@@ -88,7 +90,8 @@ Engine::~Engine() {
   for (auto& s : stack) release_side(*s);
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
                   &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
-                  &gn_sums})
+                  &gn_sums, &sat_band, &sat, &rs_fwd, &rs_inv, &rs_out, &rs_tab, &pre_buf, &pre_hist, &pre_cdf,
+                  &pre_part, &pre_rgba})
     b->release();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
@@ -96,7 +99,7 @@ Engine::~Engine() {
 }
 
 void Engine::release_side(Side& s) {
-  for (DBuf* b : {&s.gray, &s.rank, &s.feat, &s.mag, &s.stack, &s.il}) b->release();
+  for (DBuf* b : {&s.raw, &s.gray, &s.rank, &s.feat, &s.mag, &s.stack, &s.il}) b->release();
   s.has = false;
   s.il_ok = false;
 }
@@ -121,16 +124,139 @@ const float2* Engine::twiddles(int L) {
   return b.as<float2>();
 }
 
+int Engine::col_lines(int L) {
+  const int forced = FftTune::cols;
+  if (forced) return forced;
+  if (!set.b("fft_autotune")) return 0;
+  static std::mutex mu;
+  static std::map<int, int> best;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = best.find(L);
+    if (it != best.end()) return it->second;
+  }
+  const int c = autotune_cols(L);
+  std::lock_guard<std::mutex> lk(mu);
+  best[L] = c;
+  return c;
+}
+
+// Times every column tiling that launches at length L (plain 1–16 columns per workgroup, two
+// groups sharing the LDS, streaming over planes) on square planes of ≥ 64 MB, three runs each
+// after a warm-up, and returns the fastest (0 keeps the built-in default when none is faster).
+// Every tiling computes each column with the same stages, so the choice never changes results.
+int Engine::autotune_cols(int L) {
+  const int nx = L;
+  const size_t plane = (size_t)nx * L * 8;
+  const int planes = (int)std::min<size_t>(16, std::max<size_t>(1, ((size_t)64 << 20) / plane));
+  tune_buf.ensure(plane * planes);
+  ZC_HIP(hipMemsetAsync(tune_buf.p, 0, plane * planes, st));
+  const float2* tw = twiddles(L);
+  hipEvent_t e0, e1;
+  ZC_HIP(hipEventCreate(&e0));
+  ZC_HIP(hipEventCreate(&e1));
+  auto time = [&](int lines) -> double {
+    if (!launch_colfft(tune_buf.as<float2>(), planes, nx, L, L, true, tw, st, lines)) return INFINITY;
+    float best = INFINITY;
+    for (int r = 0; r < 3; ++r) {
+      ZC_HIP(hipEventRecord(e0, st));
+      launch_colfft(tune_buf.as<float2>(), planes, nx, L, L, true, tw, st, lines);
+      ZC_HIP(hipEventRecord(e1, st));
+      ZC_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      ZC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  double t_best = time(0);
+  int c_best = 0;
+  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80}) {
+    const double t = time(lines);
+    if (t < 0.97 * t_best) {
+      t_best = t;
+      c_best = lines;
+    }
+  }
+  ZC_HIP(hipEventDestroy(e0));
+  ZC_HIP(hipEventDestroy(e1));
+  tune_buf.release();
+  return c_best;
+}
+
 void Engine::need_images() {
   if (!has_images()) throw std::runtime_error("NoImages");
+  for (Side& sd : sides)
+    if (sd.baked != bake_key(sd)) bake(sd);
+}
+
+std::string Engine::bake_key(const Side& sd) const {
+  std::string k = set.e("smi_features") + (set.b("half") ? "|h" : "|f");
+  if (sd.preprocess) {
+    char b[160];
+    std::snprintf(b, sizeof b, "|c%d,%u,%u|b%d,%.17g,%.17g|i%d", set.b("clahe") ? 1 : 0, set.u("clahe_grid"),
+                  set.u("clahe_bins"), set.b("band") ? 1 : 0, set.f("bp_fine"), set.f("bp_coarse"),
+                  set.b("invert") ? 1 : 0);
+    k += b;
+  }
+  return k;
+}
+
+void Engine::bake(Side& sd) {
+  const int w = sd.w, h = sd.h;
+  const int64_t n = (int64_t)w * h;
+  const bool half = set.b("half");
+  float* raw = sd.raw.as<float>();
+  float* work = sd.gray.as<float>();
+  if (!sd.preprocess) {
+    LAUNCH("store", launch_store(raw, work, n, half, st));
+  } else {
+    // five scratch planes: the work image in progress, fine, coarse and two for the passes
+    pre_buf.ensure((size_t)n * 4 * 5);
+    float* A = pre_buf.as<float>();
+    float* Bf = A + n;
+    float* Dc = Bf + n;
+    float* P0 = Dc + n;
+    float* T = P0 + n;
+    if (set.b("clahe")) {
+      const ClaheArgs a = clahe_args(w, h, 40.0, set.u("clahe_bins"), set.u("clahe_grid"));
+      pre_hist.ensure((size_t)a.hg * a.hg * a.nb * 4);
+      pre_cdf.ensure((size_t)a.hg * a.hg * a.nb * 4);
+      LAUNCH("clahe", launch_clahe(raw, a, pre_hist.as<uint32_t>(), pre_cdf.as<float>(), A, half, st));
+    } else {
+      LAUNCH("store", launch_store(raw, A, n, half, st));
+    }
+    if (set.b("band")) {
+      // gls.zig:519-545: fine = fine_n Gaussian passes, coarse = coarse_n − fine_n more on top
+      auto rnd = [](double v) { return std::floor(v + 0.5); };
+      const double fp = set.f("bp_fine"), cp = set.f("bp_coarse");
+      const int fine_n = (int)std::max(1.0, std::min(16.0, rnd(std::pow(fp > 0 ? fp : 1.0, 2.0))));
+      const int coarse_n = std::max(fine_n + 1, (int)std::min(64.0, rnd(std::pow(cp > 0 ? cp : 6.0, 2.0))));
+      // passes of (x, y) Gaussian passes from src to dst, intermediates alternating p / q
+      auto blur = [&](const float* src, float* dst, int passes, float* p, float* q) {
+        const float* from = src;
+        for (int i = 0; i < passes; ++i) {
+          float* to = i == passes - 1 ? dst : (i % 2 == 0 ? p : q);
+          LAUNCH("gauss", launch_gauss(from, T, w, h, 1, half, st));
+          LAUNCH("gauss", launch_gauss(T, to, w, h, 0, half, st));
+          from = to;
+        }
+      };
+      blur(A, Bf, fine_n, P0, Dc);
+      blur(Bf, Dc, coarse_n - fine_n, P0, A);   // A holds nothing needed any more
+      pre_part.ensure((size_t)band_parts(n) * 8);
+      LAUNCH("band", launch_band_finish(Bf, Dc, P0, pre_part.as<float>(), A, n, half, st));
+    }
+    if (set.b("invert")) LAUNCH("invert", launch_invert(A, n, half, st));
+    ZC_HIP(hipMemcpyAsync(work, A, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  sd.il_ok = false;
+  prepare(sd, true);
+  sd.baked = bake_key(sd);
 }
 
 void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) {
   if (s < 0 || s > 1 || w < 8 || h < 8 || !data) throw std::runtime_error("BadImage");
-  if (preprocess && (set.b("clahe") || set.b("band") || set.b("invert")))
-    throw std::runtime_error(
-        "Unsupported: preprocessing (CLAHE / band pass / invert) is not part of this library; pass "
-        "preprocess=False or configure clahe=False, band=False, invert=False");
   Side& sd = sides[s];
   const size_t n = (size_t)w * h;
   sd.w = w;
@@ -138,14 +264,39 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
   sd.has = false;
   sd.il_ok = false;
   sd.preprocess = preprocess;
+  sd.raw.ensure(n * 4);
   sd.gray.ensure(n * 4);
   sd.rank.ensure(n * 4);
   sd.feat.ensure(n * 16);
   sd.mag.ensure(n * 4);
   sd.stack.ensure(n * 20);
   // host or device memory (the caller finishes any device work that writes `data` first)
-  ZC_HIP(hipMemcpyAsync(sd.gray.p, data, n * 4, hipMemcpyDefault, st));
-  prepare(sd, true);
+  ZC_HIP(hipMemcpyAsync(sd.raw.p, data, n * 4, hipMemcpyDefault, st));
+  bake(sd);
+  sd.has = true;
+}
+
+// exports.zig:60-63 / engine.zig:422-427: RGBA8 → gray (0.299 R + 0.587 G + 0.114 B), then the
+// work image always preprocessed per the settings.
+void Engine::set_image_rgba(int s, const uint8_t* rgba, int w, int h) {
+  if (s < 0 || s > 1 || w < 8 || h < 8 || !rgba) throw std::runtime_error("BadImage");
+  const size_t n = (size_t)w * h;
+  pre_rgba.ensure(n * 4);
+  ZC_HIP(hipMemcpyAsync(pre_rgba.p, rgba, n * 4, hipMemcpyHostToDevice, st));
+  Side& sd = sides[s];
+  sd.raw.ensure(n * 4);
+  LAUNCH("rgba_gray", launch_rgba_gray(pre_rgba.as<uint32_t>(), sd.raw.as<float>(), (int64_t)n, st));
+  sd.w = w;
+  sd.h = h;
+  sd.has = false;
+  sd.il_ok = false;
+  sd.preprocess = true;
+  sd.gray.ensure(n * 4);
+  sd.rank.ensure(n * 4);
+  sd.feat.ensure(n * 16);
+  sd.mag.ensure(n * 4);
+  sd.stack.ensure(n * 20);
+  bake(sd);
   sd.has = true;
 }
 
@@ -342,8 +493,8 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   bool ok = true;
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.b_n_cplx, nx, ny, ch, false, y_tw, st, FftTune::cols));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   SatView sv{};
   if (rect) {
     SatArgs sa{};
@@ -363,7 +514,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     sv.by1 = std::min(Hh, -m.fr.oy + B.h);
   }
   LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
-  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, FftTune::cols));
+  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
   LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   const uint32_t floor_n = min_overlap > 0 ? (uint32_t)std::floor(min_overlap * (double)cw * (double)ch) : 0u;

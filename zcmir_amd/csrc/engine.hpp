@@ -133,6 +133,8 @@ struct Side {
   int w = 0, h = 0;
   bool has = false;
   bool preprocess = false;
+  DBuf raw;          // the image as given (f32 gray), kept to bake again when settings change
+  std::string baked; // the settings the work image and features were made with
   DBuf gray, rank, feat, mag, stack;
   DBuf il;   // features interleaved per pixel (float4) for the sweep's gathers
   bool il_ok = false;
@@ -232,7 +234,15 @@ class Engine {
 
   void set_image(int side, const float* data, int w, int h, bool preprocess);
   bool has_images() const { return sides[0].has && sides[1].has; }
+  // Images ready: re-bakes a side whose work image or features were made with other settings
+  // (engine.zig:296-303 needImages → ensureBaked / syncFeatures).
   void need_images();
+  // The settings a side's work image and features depend on.
+  std::string bake_key(const Side& sd) const;
+  // Work image (preprocessing per the settings when sd.preprocess) from sd.raw, then features
+  // (engine.zig:321-340 bake).
+  void bake(Side& sd);
+  void set_image_rgba(int s, const uint8_t* rgba, int w, int h);
   double pair_area() const { return pair_area(sides[0], sides[1]); }
   double pair_area(const Side& A, const Side& B) const;
 
@@ -315,10 +325,16 @@ class Engine {
   DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
   DBuf sat_band, sat;   // rectangle plans: band sums and summed-area tables
   DBuf rs_fwd, rs_inv, rs_out, rs_tab;   // roto-scale maps: both directions, finished, log-polar tables
+  DBuf pre_buf, pre_hist, pre_cdf, pre_part, pre_rgba;   // preprocessing scratch
   DBuf sw_poses, sw_pose_fixed, sw_zero, sw_ns, sw_part, sw_peaks;
   DBuf gn_jobs, gn_part, gn_sums;
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
+  // Column-FFT tiling for column length L (launch_colfft's `lines`): zc_fft_tune's value when set,
+  // else (fft_autotune) the fastest of the tilings timed once per length and process, else 0.
+  int col_lines(int L);
+  int autotune_cols(int L);
+  DBuf tune_buf;
   void ensure_acf(Side& sd);
   void ensure_il(Side& sd);
 

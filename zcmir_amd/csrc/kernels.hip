@@ -922,63 +922,77 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
 // When the fixed side's weight plane is the indicator of the rectangle R (the fixed image pasted
 // at full resolution), correlation (a, 0) at lag t is the sum of plane a over R + t clipped to the
 // canvas: four reads of plane a's summed-area table S(y, x) = sum over rows < y, columns < x
-// ((H+1) x (W+1) doubles per plane). The tables are built in three passes: per-band column sums
-// (k_sat_band), their exclusive prefix over bands (k_sat_carry), then every band's rows with a
-// running column carry (k_sat_rows). Canvas planes are read twice; the tables written once.
-constexpr int kSatBand = 16;   // rows per band
+// ((H+1) x (W+1) doubles per plane). The tables are built in three passes over bands of kSatBand
+// rows: every table's column sums per band (k_sat_band: one thread per column, all tables from
+// one read of the canvas), their exclusive prefix over bands (k_sat_carry), then every
+// (band, table) by a workgroup that scans its rows with a running column carry (k_sat_rows).
+constexpr int kSatBand = 8;   // rows per band
 
-__device__ __forceinline__ double real_plane_d(int pl, const float* s) {
+// The real planes of a canvas stack sample in FP64: w, w f_i, and w f_i f_j = s_i s_j / s_0
+// (0 where s_0 <= 1e-6), the last with one reciprocal per sample.
+__device__ __forceinline__ double real_plane_d(int pl, const float* s, double inv0) {
   if (pl < 5) return (double)s[pl];
   if (s[0] <= 1e-6f) return 0.0;
   constexpr int TI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
   constexpr int TJ[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
   const int k = pl - 5;
-  return (double)s[1 + TI[k]] * (double)s[1 + TJ[k]] / (double)s[0];
+  return (double)s[1 + TI[k]] * (double)s[1 + TJ[k]] * inv0;
 }
 
-__device__ __forceinline__ double canvas_plane_d(const float* __restrict__ canvas, int64_t n, int64_t at, int pl) {
-  float s[5];
+__device__ __forceinline__ double canvas_inv0(const float* s) { return s[0] > 1e-6f ? 1.0 / (double)s[0] : 0.0; }
+
+__device__ __forceinline__ void canvas_values(const float* __restrict__ canvas, int64_t n, int64_t at, float* s) {
 #pragma unroll
   for (int p = 0; p < 5; ++p) s[p] = canvas[p * n + at];
-  return real_plane_d(pl, s);
 }
 
-// blockIdx.x band, blockIdx.y table: band[(k · nb + b) · W + x] = column sums of the band's rows.
+// blockIdx.x band, blockIdx.y a chunk of 256 columns: band[(k · nb + b) · W + x] = column sums of
+// table k's plane over the band's rows, for every table.
 __global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canvas, int W, int H, SatArgs sa,
-                                                  double* band) {
-  const int b = blockIdx.x, k = blockIdx.y, nb = gridDim.x, pl = sa.plane[k];
+                                                  int n_tab, double* band) {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int x = blockIdx.y * 256 + threadIdx.x;
+  if (x >= W) return;
   const int64_t n = (int64_t)W * H;
   const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
-  for (int x = threadIdx.x; x < W; x += 256) {
-    double acc = 0.0;
-    for (int y = y0; y < y1; ++y) acc += canvas_plane_d(canvas, n, (int64_t)y * W + x, pl);
-    band[((int64_t)k * nb + b) * W + x] = acc;
+  double acc[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+  for (int y = y0; y < y1; ++y) {
+    float sv[5];
+    canvas_values(canvas, n, (int64_t)y * W + x, sv);
+    const double inv0 = canvas_inv0(sv);
+#pragma unroll
+    for (int k = 0; k < 15; ++k)
+      if (k < n_tab) acc[k] += real_plane_d(sa.plane[k], sv, inv0);
   }
+#pragma unroll
+  for (int k = 0; k < 15; ++k)
+    if (k < n_tab) band[((int64_t)k * nb + b) * W + x] = acc[k];
 }
 
-// In place: band sums → exclusive prefix over bands, per (table, column).
-__global__ __launch_bounds__(256) void k_sat_carry(double* band, int W, int nb, int n_tab) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (int64_t)n_tab * W) return;
-  const int k = (int)(i / W), x = (int)(i - (int64_t)k * W);
+// In place: band sums → exclusive prefix over bands, per (table, column). blockIdx.x: table ·
+// column chunks of 64; the block's 4 waves take a quarter of the bands each.
+__global__ __launch_bounds__(256) void k_sat_carry(double* band, int W, int nb) {
+  __shared__ double seg[4][64];
+  const int chunks = (W + 63) / 64;
+  const int k = blockIdx.x / chunks, x = (blockIdx.x - k * chunks) * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
+  const int per = (nb + 3) / 4, b0 = q * per, b1 = min(nb, b0 + per);
   double* col = band + (int64_t)k * nb * W + x;
+  double tot = 0.0;
+  if (x < W)
+    for (int b = b0; b < b1; ++b) tot += col[(int64_t)b * W];
+  seg[q][threadIdx.x & 63] = tot;
+  __syncthreads();
   double acc = 0.0;
-  int b = 0;
-  for (; b + 8 <= nb; b += 8) {
-    double v[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = col[(int64_t)(b + j) * W];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      col[(int64_t)(b + j) * W] = acc;
-      acc += v[j];
+  for (int j = 0; j < q; ++j) acc += seg[j][threadIdx.x & 63];
+  if (x < W)
+    for (int b = b0; b < b1; ++b) {
+      const double v = col[(int64_t)b * W];
+      col[(int64_t)b * W] = acc;
+      acc += v;
     }
-  }
-  for (; b < nb; ++b) {
-    const double v = col[(int64_t)b * W];
-    col[(int64_t)b * W] = acc;
-    acc += v;
-  }
 }
 
 // Exclusive prefix of v over the 256 threads of the block (lds: 4 doubles).
@@ -998,50 +1012,52 @@ __device__ __forceinline__ double block_excl_scan(double v, double* lds) {
   return off + inc - v;
 }
 
-// blockIdx.x band, blockIdx.y table; thread t owns columns [t·E, t·E + E), E = ceil(W / 256) <= 8.
+// blockIdx.x band, blockIdx.y table; thread t owns columns [t·E, t·E + E).
+template <int E>
 __global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canvas, int W, int H, SatArgs sa,
                                                   const double* __restrict__ band, double* sat) {
-  constexpr int EMAX = 8;
   __shared__ double lds[4];
   const int b = blockIdx.x, k = blockIdx.y, nb = gridDim.x, pl = sa.plane[k];
-  const int E = (W + 255) / 256;
   const int xs = threadIdx.x * E;
   const int64_t n = (int64_t)W * H, SW = W + 1;
   double* S = sat + (int64_t)k * SW * (H + 1);
   // column carry cc[e] = S(y, xs + e + 1): the row prefix of the earlier bands' column sums
-  double cc[EMAX];
+  double cc[E];
   {
     const double* crow = band + ((int64_t)k * nb + b) * W;
     double tot = 0.0;
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e) {
-      cc[e] = (e < E && xs + e < W) ? crow[xs + e] : 0.0;
-      tot += cc[e];
+    for (int e = 0; e < E; ++e) {
+      tot += xs + e < W ? crow[xs + e] : 0.0;
       cc[e] = tot;
     }
     const double off = block_excl_scan(tot, lds);
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e) cc[e] += off;
+    for (int e = 0; e < E; ++e) cc[e] += off;
   }
-  const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
-  if (b == 0) {
+  if (b == 0)
     for (int x = threadIdx.x; x <= W; x += 256) S[x] = 0.0;
-  }
+  const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
   for (int y = y0; y < y1; ++y) {
-    double v[EMAX];
+    double v[E];
     double tot = 0.0;
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e) {
-      v[e] = (e < E && xs + e < W) ? canvas_plane_d(canvas, n, (int64_t)y * W + xs + e, pl) : 0.0;
-      tot += v[e];
+    for (int e = 0; e < E; ++e) {
+      double d = 0.0;
+      if (xs + e < W) {
+        float sv[5];
+        canvas_values(canvas, n, (int64_t)y * W + xs + e, sv);
+        d = real_plane_d(pl, sv, canvas_inv0(sv));
+      }
+      tot += d;
       v[e] = tot;
     }
     const double off = block_excl_scan(tot, lds);
     double* row = S + (int64_t)(y + 1) * SW;
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e) {
+    for (int e = 0; e < E; ++e) {
       cc[e] += off + v[e];
-      if (e < E && xs + e < W) row[xs + e + 1] = cc[e];
+      if (xs + e < W) row[xs + e + 1] = cc[e];
     }
     if (threadIdx.x == 0) row[0] = 0.0;
   }
@@ -1053,10 +1069,11 @@ bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int 
                       hipStream_t st) {
   if (W < 1 || H < 1 || W > 256 * 8 || n_tab < 1 || n_tab > 15) return false;
   const int nb = sat_bands(H);
-  k_sat_band<<<dim3(nb, n_tab), 256, 0, st>>>(canvas, W, H, sa, band);
-  const int64_t nt = (int64_t)n_tab * W;
-  k_sat_carry<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(band, W, nb, n_tab);
-  k_sat_rows<<<dim3(nb, n_tab), 256, 0, st>>>(canvas, W, H, sa, band, sat);
+  k_sat_band<<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, sa, n_tab, band);
+  k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 256, 0, st>>>(band, W, nb);
+  const dim3 g(nb, n_tab);
+  if (W <= 256 * 4) k_sat_rows<4><<<g, 256, 0, st>>>(canvas, W, H, sa, band, sat);
+  else k_sat_rows<8><<<g, 256, 0, st>>>(canvas, W, H, sa, band, sat);
   return true;
 }
 
@@ -1821,9 +1838,16 @@ void launch_rs_finish(const float* fwd, const float* inv, int n, bool mirror, do
 
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
                    hipStream_t st, int lines) {
+  // a tiling that cannot launch at this width (nx not a multiple of its columns) falls back to
+  // the default tiling
 #define CALL(LL)                                                                                   \
-  return inverse ? colfft_L<LL, 1>(data, planes, nx, rows_in, 1.0f, tw_y, lines, st)               \
-                 : colfft_L<LL, -1>(data, planes, nx, rows_in, 1.0f, tw_y, lines, st);
+  {                                                                                                \
+    auto run = [&](int ln) {                                                                       \
+      return inverse ? colfft_L<LL, 1>(data, planes, nx, rows_in, 1.0f, tw_y, ln, st)              \
+                     : colfft_L<LL, -1>(data, planes, nx, rows_in, 1.0f, tw_y, ln, st);            \
+    };                                                                                             \
+    return run(lines) || (lines != 0 && run(0));                                                   \
+  }
   ZC_POW2_SWITCH(ny, CALL)
 #undef CALL
   return true;

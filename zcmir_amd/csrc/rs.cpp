@@ -97,10 +97,10 @@ Engine::RsPlan Engine::rs_map_dir(Side& A, Side& B, const Mat3& Hp, double c0, d
   bool ok = true;
   LAUNCH("pack_logpolar", ok &= launch_pack_logpolar(plan, la, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
   LAUNCH("pack_logpolar", ok &= launch_pack_logpolar(plan, lb, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, p.n_lam, false, y_tw, st, FftTune::cols));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, p.n_lam, false, y_tw, st, FftTune::cols));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, p.n_lam, false, y_tw, st, col_lines(ny)));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, nx, ny, p.n_lam, false, y_tw, st, col_lines(ny)));
   LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
-  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, FftTune::cols));
+  LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
   LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   CombineArgs ca{};

@@ -55,6 +55,8 @@ static const SettingDef kSettings[] = {
     {"invert", Kind::Bool, "false", nullptr},
     {"half", Kind::Bool, "true", nullptr},
     {"fft_sizes", Kind::Enum, "ladder", "ladder|compact"},
+    // this library's: time the column-FFT tilings of a length at its first use, keep the fastest
+    {"fft_autotune", Kind::Bool, "true", nullptr},
     {"detector", Kind::Enum, "pos_gift", "pos_gift|gls_mift"},
     {"n_octaves", Kind::U32, "3", nullptr},
     {"max_points", Kind::U32, "2000", nullptr},

@@ -208,7 +208,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     LAUNCH("pack_sweep_fixed", ok &= launch_pack_sweep(plan, B.il.as<float4>(), sw_pose_fixed.as<SweepPose>(), 1, uf,
                                                         (int)nx, spec_b.as<float2>(), tx, st));
     LAUNCH("colfft_fwd_fixed", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, (int)nx, (int)ny, (int)Fy, false,
-                                                    ty, st, FftTune::cols));
+                                                    ty, st, col_lines((int)ny)));
     // moving windows: as wide as the plane allows without wraparound
     SweepPack um{(float)ox, (float)oy, (float)cell, (int)grid, A.w, A.h, (int)(nx - Fx + 1), (int)(ny - Fy + 1), (int)ny};
     for (uint32_t c0 = k0; c0 < k1; c0 += Bn) {
@@ -224,11 +224,11 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
       LAUNCH("pack_sweep", ok &= launch_pack_sweep(plan, A.il.as<float4>(), dposes + c0, b, um, (int)nx,
                                                     spec_a.as<float2>(), tx, st));
       LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx * b, (int)nx, (int)ny, um.win_y, false,
-                                              ty, st, FftTune::cols));
+                                              ty, st, col_lines((int)ny)));
       LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), (int)nx, (int)ny,
                                  st, b));
       LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out * b, (int)nx, (int)ny, (int)ny, true, ty,
-                                              st, FftTune::cols));
+                                              st, col_lines((int)ny)));
       LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), (int)nx,
                                                              (int)ny, tx, st, b));
       LAUNCH("combine_sweep", launch_combine_sweep(plan, corr.as<float>(), sw_ns.as<float>(), nn, b, ca,

@@ -193,13 +193,14 @@ def propose(net, moving, fixed, k=8):
     return out
 
 
-def fine_cells(cells, n_theta=72):
+def fine_cells(cells, n_theta=72, margin=0):
     """The (rotation, scale) cells of an n_theta-rotation sweep grid (same 9 scales) inside the
-    proposed 15° cells: each coarse rotation index covers the fine rotations within ±7.5°."""
+    proposed 15° cells: each coarse rotation index covers the fine rotations within ±7.5°, and
+    `margin` more fine rotations on each side (a pose near a coarse cell's edge)."""
     per = n_theta // N_TH
     out = []
     for it, is_ in cells:
-        for d in range(-(per // 2), per - per // 2):
+        for d in range(-(per // 2) - margin, per - per // 2 + margin):
             c = ((per * it + d) % n_theta, is_)
             if c not in out:
                 out.append(c)
@@ -237,17 +238,17 @@ def calibrate(reg, net, moving, fixed, poses, thetas, sigmas, q=0.05):
     return float(np.quantile(vals, q))
 
 
-def learned_search(reg, net, moving, fixed, k=8, threshold=0.0):
+def learned_search(reg, net, moving, fixed, k=8, threshold=0.0, margin=1):
     """The search with the prior: the network's top-k classes (2k cells: θ and θ + 180°), the sweep
-    over the grid cells inside them (the registrar's sweep settings: 72 rotations × 9 scales), NMS
-    and Gauss–Newton. When the proposals' check_score falls below 0.8 × threshold (calibrate), the
+    over the grid cells inside them and `margin` fine rotations beyond (the registrar's sweep
+    settings: 72 rotations × 9 scales), NMS and Gauss–Newton. When the proposals' check_score falls below 0.8 × threshold (calibrate), the
     full search runs from the same start instead. moving, fixed: CUDA tensors [h, w] of the
     registrar's images.
 
     Returns (Search, info dict: proposed cells, fine cells, check score, fell_back)."""
     H0 = reg.pose
     cells = propose(net, moving[None], fixed[None], k)[0]
-    fine = fine_cells(cells, int(reg.settings["sw_nth"]))
+    fine = fine_cells(cells, int(reg.settings["sw_nth"]), margin)
     score = check_score(reg, fine)
     fell_back = score < 0.8 * threshold
     reg.pose = H0
@@ -283,6 +284,30 @@ def truth_cell(th, sg):
     it = int(round((((th + 180.0) % 360.0)) / 15.0)) % N_TH
     is_ = int(min(max(round(4 * math.log(sg) / math.log(1.25) + 4), 0), N_SG - 1))
     return it, is_
+
+
+def gpu_textures(n, size, seed, n_blobs=20):
+    """n training textures on the GPU, the recipe of synth._texture (Gaussian-filtered noise,
+    sigma 4, periodic; unit spread; n_blobs Gaussian blobs; scaled to [0, 1]), each its own draw."""
+    import torch
+
+    g = torch.Generator(device="cuda").manual_seed(int(seed))
+    x = torch.randn(n, size, size, device="cuda", generator=g)
+    f = torch.fft.fftfreq(size, device="cuda", dtype=torch.float32)
+    k = torch.exp(-2.0 * math.pi ** 2 * 16.0 * (f[:, None] ** 2 + f[None, :] ** 2))
+    x = torch.fft.ifft2(torch.fft.fft2(x) * k).real
+    x = x / x.flatten(1).std(dim=1).clamp_min(1e-12)[:, None, None]
+    yy = torch.arange(size, device="cuda", dtype=torch.float32)[None, :, None]
+    xx = torch.arange(size, device="cuda", dtype=torch.float32)[None, None, :]
+    u = lambda lo, hi: lo + (hi - lo) * torch.rand(n, 1, 1, device="cuda", generator=g)
+    for _ in range(n_blobs):
+        cy, cx = u(0.0, size), u(0.0, size)
+        sg = u(2.0, max(3.0, size / 12.0))
+        amp = u(-3.0, 3.0)
+        x = x + amp * torch.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg * sg))
+    x = x - x.flatten(1).min(dim=1).values[:, None, None]
+    x = x / x.flatten(1).max(dim=1).values.clamp_min(1e-12)[:, None, None]
+    return x.contiguous()
 
 
 def make_pairs(textures, idx, Hs, remaps):
@@ -326,21 +351,19 @@ def train(n_pairs=16000, n_val=1000, size=512, epochs=12, batch=128, lr=1e-3, se
     import torch
 
     import zcmir_amd
-    from . import synth
 
     rng = np.random.default_rng(seed)
     t0 = time.time()
-    tex = np.stack([synth._texture(size, size, rng) for _ in range(96)]).astype(np.float32)
-    textures = torch.tensor(tex, device="cuda")
-    log(f"textures {time.time() - t0:.1f} s")
     reg = zcmir_amd.Registrar(half=False)
     total = n_pairs + n_val
     Hs, th, sg, rm = random_poses(rng, total, size)
-    idx = rng.integers(0, len(tex), total)
     X, Y, agree, norms = [], [], [], []
     for s0 in range(0, total, 500):
         s1 = min(total, s0 + 500)
-        mv, fx = make_pairs(textures, idx[s0:s1], Hs[s0:s1], rm[s0:s1])
+        # every pair its own texture (validation pairs included: held-out textures)
+        textures = gpu_textures(s1 - s0, size, seed * 1000003 + s0)
+        mv, fx = make_pairs(textures, np.arange(s1 - s0), Hs[s0:s1], rm[s0:s1])
+        del textures
         lab, bn = sweep_labels(reg, mv, fx)
         X.append(features(mv, fx))
         Y.append(lab)
