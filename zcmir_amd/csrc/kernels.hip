@@ -784,9 +784,9 @@ __device__ __forceinline__ bool chol(double (&g)[4][4], double (&rd)[4]) {
       for (int k = 0; k < j; ++k) s -= g[i][k] * g[j][k];
       if (i == j) {
         if (!(s > 0.0)) return false;
-        const double l = sqrt(s);
-        g[i][i] = l;
-        rd[i] = 1.0 / l;
+        const double r = rsqrt(s);   // 1 / L_ii without a division
+        g[i][i] = s * r;
+        rd[i] = r;
       } else {
         g[i][j] = s * rd[j];
       }
