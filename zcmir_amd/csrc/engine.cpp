@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <sstream>
 
 namespace zc {
@@ -29,6 +30,10 @@ hipEvent_t Prof::get() {
   return e;
 }
 void Prof::begin(hipStream_t st) {
+  if (last_) {
+    cur_ = last_;
+    return;
+  }
   cur_ = get();
   ZC_HIP(hipEventRecord(cur_, st));
 }
@@ -37,16 +42,19 @@ void Prof::end(const char* name, hipStream_t st) {
   ZC_HIP(hipEventRecord(b, st));
   pending_.push_back({name, cur_, b});
   cur_ = nullptr;
+  last_ = b;
 }
 void Prof::collect() {
+  // an event may end one launch and start the next: return each to the pool once
+  std::set<hipEvent_t> seen;
   for (auto& p : pending_) {
     float ms = 0;
     ZC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
     auto& a = acc_[p.name];
     a.first += 1;
     a.second += ms;
-    pool_.push_back(p.a);
-    pool_.push_back(p.b);
+    for (hipEvent_t e : {p.a, p.b})
+      if (e != last_ && seen.insert(e).second) pool_.push_back(e);
   }
   pending_.clear();
 }
@@ -108,6 +116,7 @@ void Engine::sync() {
   ZC_HIP(hipStreamSynchronize(st));
   ++counts[3];
   if (prof.on) prof.collect();
+  prof.break_chain();
 }
 
 const float2* Engine::twiddles(int L) {
@@ -151,6 +160,7 @@ int Engine::autotune_cols(int L) {
   const int planes = (int)std::min<size_t>(16, std::max<size_t>(1, ((size_t)64 << 20) / plane));
   tune_buf.ensure(plane * planes);
   ZC_HIP(hipMemsetAsync(tune_buf.p, 0, plane * planes, st));
+  prof.break_chain();
   const float2* tw = twiddles(L);
   hipEvent_t e0, e1;
   ZC_HIP(hipEventCreate(&e0));
@@ -181,6 +191,7 @@ int Engine::autotune_cols(int L) {
   ZC_HIP(hipEventDestroy(e0));
   ZC_HIP(hipEventDestroy(e1));
   tune_buf.release();
+  prof.break_chain();
   return c_best;
 }
 
@@ -249,6 +260,7 @@ void Engine::bake(Side& sd) {
     }
     if (set.b("invert")) LAUNCH("invert", launch_invert(A, n, half, st));
     ZC_HIP(hipMemcpyAsync(work, A, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    prof.break_chain();
   }
   sd.il_ok = false;
   prepare(sd, true);
@@ -272,6 +284,7 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
   sd.stack.ensure(n * 20);
   // host or device memory (the caller finishes any device work that writes `data` first)
   ZC_HIP(hipMemcpyAsync(sd.raw.p, data, n * 4, hipMemcpyDefault, st));
+  prof.break_chain();
   bake(sd);
   sd.has = true;
 }
@@ -283,6 +296,7 @@ void Engine::set_image_rgba(int s, const uint8_t* rgba, int w, int h) {
   const size_t n = (size_t)w * h;
   pre_rgba.ensure(n * 4);
   ZC_HIP(hipMemcpyAsync(pre_rgba.p, rgba, n * 4, hipMemcpyHostToDevice, st));
+  prof.break_chain();
   Side& sd = sides[s];
   sd.raw.ensure(n * 4);
   LAUNCH("rgba_gray", launch_rgba_gray(pre_rgba.as<uint32_t>(), sd.raw.as<float>(), (int64_t)n, st));
@@ -313,9 +327,11 @@ void Engine::prepare(Side& sd, bool acf) {
   tmp_i64.ensure(8 * 4);
   uint32_t* acc32 = tmp_u32.as<uint32_t>();
   ZC_HIP(hipMemsetAsync(acc32, 0, 4, st));
+  prof.break_chain();
   LAUNCH("maxabs", launch_maxabs(gray, n, acc32, st));
   uint32_t maxbits = 0;
   ZC_HIP(hipMemcpyAsync(&maxbits, acc32, 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   sync();
   float maxabs;
   std::memcpy(&maxabs, &maxbits, 4);
@@ -325,9 +341,11 @@ void Engine::prepare(Side& sd, bool acf) {
     long long* hi = tmp_i64.as<long long>();
     unsigned long long* lo = reinterpret_cast<unsigned long long*>(hi + 1);
     ZC_HIP(hipMemsetAsync(hi, 0, 16, st));
+    prof.break_chain();
     LAUNCH("fixsum", launch_fixsum(gray, n, std::ldexp(1.0, K), mean, squared, hi, lo, st));
     long long h2[2];
     ZC_HIP(hipMemcpyAsync(h2, hi, 16, hipMemcpyDeviceToHost, st));
+    prof.break_chain();
     sync();
     const __int128 S = (__int128)h2[0] * ((__int128)1 << 32) + (__int128)(unsigned long long)h2[1];
     const long long shi = (long long)(S >> 64);
@@ -343,8 +361,10 @@ void Engine::prepare(Side& sd, bool acf) {
   uint32_t* hist = acc32 + 16;
   uint32_t* cdf = acc32 + 16 + 272;
   ZC_HIP(hipMemsetAsync(hist, 0, 257 * 4, st));
+  prof.break_chain();
   LAUNCH("hist", launch_hist(gray, n, sd.mean, sd.spread, hist, st));
   ZC_HIP(hipMemcpyAsync(sd.hist, hist, 257 * 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   sync();
   uint64_t accu = 0;
   for (int i = 0; i < 256; ++i) {
@@ -354,6 +374,7 @@ void Engine::prepare(Side& sd, bool acf) {
   sd.cdf[256] = (uint32_t)accu;
   sd.cdf[257] = 0;
   ZC_HIP(hipMemcpyAsync(cdf, sd.cdf, 258 * 4, hipMemcpyHostToDevice, st));
+  prof.break_chain();
   LAUNCH("rank", launch_rank(gray, n, sd.mean, sd.spread, hist, cdf, sd.rank.as<float>(), st));
   // features and their moments
   const int mode = sd.mode == "laplacian" ? 0 : (sd.mode == "three" ? 1 : 2);
@@ -365,6 +386,7 @@ void Engine::prepare(Side& sd, bool acf) {
   LAUNCH("sum_parts", launch_sum_parts(parts.as<double>(), nb, 14, sums.as<double>(), st));
   double m[14];
   ZC_HIP(hipMemcpyAsync(m, sums.p, 14 * 8, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   sync();
   // global ridge whitening (smi.wgsl:554-599), FP64
   const double nn = (double)n;
@@ -408,6 +430,7 @@ void Engine::ensure_acf(Side& sd) {
   LAUNCH("sum_parts", launch_sum_parts(parts.as<double>(), nb, 1156, sums.as<double>(), st));
   std::vector<double> raw(1156);
   ZC_HIP(hipMemcpyAsync(raw.data(), sums.p, 1156 * 8, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   sync();
   for (int lag = 0; lag < 289; ++lag) {
     const int dx = lag % 17 - 8, dy = lag / 17 - 8;
@@ -529,6 +552,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st, rect ? &sv : nullptr));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   ++counts[1];
   if (wait) {
@@ -542,6 +566,7 @@ void Engine::read_map(float* out, size_t len, const MapOut& m) {
   const size_t nn = (size_t)m.g.nx * m.g.ny;
   if (len < nn) throw std::runtime_error("MapTooSmall");
   ZC_HIP(hipMemcpyAsync(out, map.p, nn * 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   sync();
 }
@@ -590,6 +615,7 @@ void Engine::moments(int as, int bs, const Mat3& Hp, double out[45]) {
                                              parts.as<double>(), st));
   LAUNCH("sum_parts", launch_sum_parts(parts.as<double>(), nb, 45, sums.as<double>(), st));
   ZC_HIP(hipMemcpyAsync(out, sums.p, 45 * 8, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   sync();
 }
@@ -672,10 +698,12 @@ void Engine::fft2(float* data, int n, int planes, bool inverse) {
   const size_t bytes = (size_t)planes * n * n * 8;
   work.ensure(bytes);
   ZC_HIP(hipMemcpyAsync(work.p, data, bytes, hipMemcpyHostToDevice, st));
+  prof.break_chain();
   bool ok = true;
   LAUNCH("fft2", ok = launch_fft2(work.as<float2>(), planes, n, inverse, twiddles(n), st, FftTune::cols));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   ZC_HIP(hipMemcpyAsync(data, work.p, bytes, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   sync();
 }
 

@@ -97,11 +97,15 @@ struct ClimbResult {
 };
 
 // Per-kernel timing with HIP events on the engine's stream (zc_profile / zc_profile_report).
+// Launches back to back share events: a launch's start is the previous launch's end event, so a
+// profiled map records one event per kernel (its duration includes the launch gap before it).
+// Anything else put on the stream (copies, host waits) breaks the chain.
 class Prof {
  public:
   bool on = false;
   void begin(hipStream_t st);
   void end(const char* name, hipStream_t st);
+  void break_chain() { last_ = nullptr; }
   void collect();   // after a synchronization
   std::string report() const;
   void reset() {
@@ -120,6 +124,7 @@ class Prof {
   std::vector<hipEvent_t> pool_;
   std::vector<Pending> pending_;
   hipEvent_t cur_ = nullptr;
+  hipEvent_t last_ = nullptr;   // the end event of the previous launch, when nothing came between
   std::map<std::string, std::pair<long, double>> acc_;
   std::map<std::string, double> bytes_;
   hipEvent_t get();

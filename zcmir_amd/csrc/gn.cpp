@@ -183,9 +183,11 @@ void Engine::moments_jobs(const std::vector<DirJob>& d, std::vector<std::array<d
   gn_part.ensure(nj * blocks * 45 * sizeof(double));
   gn_sums.ensure(nj * 45 * sizeof(double));
   ZC_HIP(hipMemcpyAsync(gn_jobs.p, jobs.data(), nj * sizeof(PoseJob), hipMemcpyHostToDevice, st));
+  prof.break_chain();
   LAUNCH("pose_moments", launch_pose_moments_jobs(gn_jobs.as<PoseJob>(), (int)nj, blocks, gn_part.as<double>(),
                                                   gn_sums.as<double>(), st));
   ZC_HIP(hipMemcpyAsync(out.data(), gn_sums.p, nj * 45 * sizeof(double), hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   sync();
 }
@@ -234,9 +236,11 @@ void Engine::tangent_jobs(const std::vector<DirJob>& d, const std::vector<Coefs>
   gn_part.ensure(nj * blocks * 44 * sizeof(double));
   gn_sums.ensure(nj * 44 * sizeof(double));
   ZC_HIP(hipMemcpyAsync(gn_jobs.p, jobs.data(), nj * sizeof(TangentJob), hipMemcpyHostToDevice, st));
+  prof.break_chain();
   LAUNCH("tangent_grad", launch_tangent_jobs(gn_jobs.as<TangentJob>(), (int)nj, blocks, gn_part.as<double>(),
                                              gn_sums.as<double>(), st));
   ZC_HIP(hipMemcpyAsync(out.data(), gn_sums.p, nj * 44 * sizeof(double), hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   sync();
 }

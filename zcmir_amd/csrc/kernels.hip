@@ -1081,6 +1081,7 @@ bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int 
 struct SatIdx {
   int64_t o00, o01, o10, o11;   // S(y0, x0), S(y0, x1), S(y1, x0), S(y1, x1)
   bool empty;
+  bool r0, c0;                  // y0 = 0 / x0 = 0: those corners lie on the table's zero row / column
 };
 
 __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
@@ -1092,6 +1093,8 @@ __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
   SatIdx q;
   const int64_t SW = v.W + 1;
   q.empty = x1 <= x0 || y1 <= y0;
+  q.r0 = y0 == 0;
+  q.c0 = x0 == 0;
   q.o00 = y0 * SW + x0;
   q.o01 = y0 * SW + x1;
   q.o10 = y1 * SW + x0;
@@ -1099,9 +1102,14 @@ __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
   return q;
 }
 
+// Corners on the zero row or column are not read (half the lags of a map have y0 = 0 or x0 = 0).
 __device__ __forceinline__ double sat_sum(const double* __restrict__ S, const SatIdx& q) {
   if (q.empty) return 0.0;
-  return (S[q.o11] - S[q.o01]) - (S[q.o10] - S[q.o00]);
+  const double s11 = S[q.o11];
+  const double s01 = q.r0 ? 0.0 : S[q.o01];
+  const double s10 = q.c0 ? 0.0 : S[q.o10];
+  const double s00 = (q.r0 || q.c0) ? 0.0 : S[q.o00];
+  return (s11 - s01) - (s10 - s00);
 }
 
 // k_combine for the rectangle plans: correlations with sat_of >= 0 come from the tables.

@@ -135,6 +135,7 @@ void Engine::rs_finish(const RsPlan& p, bool symmetric, bool mirror) {
                                        lo, hi, rs_out.as<float>(), arg_part.as<float>(), st));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, rs_out.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
 }
 
 RsResult Engine::rs_map(const Mat3& H, const double* c_opt, bool symmetric, float* map, float* fwd, float* inv,
@@ -159,8 +160,11 @@ RsResult Engine::rs_map(const Mat3& H, const double* c_opt, bool symmetric, floa
   if ((map || fwd || (inv && symmetric)) && len < (size_t)nn) throw std::runtime_error("MapTooSmall");
   rs_finish(p, symmetric, H[0] * H[4] - H[1] * H[3] > 0);
   if (map) ZC_HIP(hipMemcpyAsync(map, rs_out.p, nn * 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   if (fwd) ZC_HIP(hipMemcpyAsync(fwd, rs_fwd.p, nn * 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   if (inv && symmetric) ZC_HIP(hipMemcpyAsync(inv, rs_inv.p, nn * 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   sync();
   RsResult r{};

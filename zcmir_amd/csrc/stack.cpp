@@ -29,6 +29,7 @@ void Engine::stack_set(uint32_t n, const float* const* mv, const uint32_t* mwh, 
       sd->mag.ensure(px * 4);
       sd->stack.ensure(px * 20);
       ZC_HIP(hipMemcpyAsync(sd->gray.p, data, px * 4, hipMemcpyHostToDevice, st));
+      prof.break_chain();
       prepare(*sd, false);
       sd->has = true;
       stack.push_back(std::move(sd));

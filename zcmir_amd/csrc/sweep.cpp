@@ -180,8 +180,11 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   sw_ns.ensure(pts_max * 4);
   sw_part.ensure(sizeof(float2) * 256 * part_max);
   ZC_HIP(hipMemcpyAsync(sw_poses.p, poses.data(), sizeof(SweepPose) * nc, hipMemcpyHostToDevice, st));
+  prof.break_chain();
   ZC_HIP(hipMemcpyAsync(sw_pose_fixed.p, &pf, sizeof pf, hipMemcpyHostToDevice, st));
+  prof.break_chain();
   ZC_HIP(hipMemcpyAsync(sw_zero.p, zero.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  prof.break_chain();
   ensure_il(A);
   ensure_il(B);
   const uint32_t min_n =
@@ -242,6 +245,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   }
   std::vector<float2> raw((size_t)nc * n_peak);
   ZC_HIP(hipMemcpyAsync(raw.data(), sw_peaks.p, raw.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   ++counts[1];
   sync();
@@ -286,6 +290,7 @@ void Engine::read_sweep_map(float* out, size_t len, uint32_t dims[2]) {
   if (nn == 0) throw std::runtime_error("NoSweep");
   if (len < nn) throw std::runtime_error("MapTooSmall");
   ZC_HIP(hipMemcpyAsync(out, sw_ns.p, nn * 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
   ++counts[2];
   sync();
   dims[0] = sweep_last_nx;
