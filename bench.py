@@ -121,6 +121,7 @@ def bench_sweep(a, np, torch, zcmir_amd, synth):
             "value": round(info["nc"] * steps / wall, 2), "unit": "poses/s", "steps": steps,
             "ms_per_sweep": round(1e3 * wall / steps, 3), "candidates": info["nc"], "grid": info["grid"],
             "cell_px": round(info["cell"], 4), "largest_fft": info["n"],
+            "fft_column_tiling": {str(info["n"]): reg.fft_tiling(info["n"])},
             "roofline": _roofline(kernels),
             "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                         for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_unit"])},

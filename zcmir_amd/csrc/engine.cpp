@@ -151,7 +151,8 @@ int Engine::col_lines(int L) {
 }
 
 // Times every column tiling that launches at length L (plain 1–16 columns per workgroup, two
-// groups sharing the LDS, streaming over planes) on square planes of ≥ 64 MB, three runs each
+// groups sharing the LDS, streaming over planes, the wide tilings with 32 points per thread or a
+// split exchange) on square planes of ≥ 64 MB, three runs each
 // after a warm-up, and returns the fastest (0 keeps the built-in default when none is faster).
 // Every tiling computes each column with the same stages, so the choice never changes results.
 int Engine::autotune_cols(int L) {
@@ -181,7 +182,7 @@ int Engine::autotune_cols(int L) {
   };
   double t_best = time(0);
   int c_best = 0;
-  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80}) {
+  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47}) {
     const double t = time(lines);
     if (t < 0.97 * t_best) {
       t_best = t;

@@ -169,6 +169,63 @@ __device__ __forceinline__ void rfft(float2 (&v)[M], float2* lds, int stride, co
   rstages_from<L, M, T, LINES, LFAST, S, 0>(v, lds, stride, tw, tid);
 }
 
+// The same exchange through half the LDS: the real parts, then the imaginary parts, each through
+// one float buffer of LINES lines of `stride` (four barriers instead of two), so that two
+// workgroups of a wide tiling fit on one CU and overlap each other's exchanges with memory.
+template <int L, int M, int T, int LINES, bool LFAST, int ST>
+__device__ __forceinline__ void rexchange_split(float2 (&v)[M], float* lds, int stride, int tid) {
+  using P = RPlan<L, M>;
+  constexpr int R = P::radix(ST), NS = P::ns(ST), NB = L / R, B = M / R;
+  constexpr int R2 = P::radix(ST + 1), NB2 = L / R2, B2 = M / R2;
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      int line, j;
+      bf_index<LINES, NB, LFAST>(tid + u * T, line, j);
+      const int d = (j / NS) * NS * R + (j % NS);
+      float* dst = lds + line * stride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[pad(d + r * NS)] = part ? v[u * R + r].y : v[u * R + r].x;
+    }
+    __syncthreads();
+    float w[M];
+#pragma unroll
+    for (int u = 0; u < B2; ++u) {
+      int line, j;
+      bf_index<LINES, NB2, LFAST>(tid + u * T, line, j);
+      const float* src = lds + line * stride;
+#pragma unroll
+      for (int r = 0; r < R2; ++r) w[u * R2 + r] = src[pad(j + r * NB2)];
+    }
+    __syncthreads();
+    // the next stage's layout replaces this stage's once both parts are read
+    if (part == 0) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) v[i].x = w[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < M; ++i) v[i].y = w[i];
+    }
+  }
+}
+
+template <int L, int M, int T, int LINES, bool LFAST, int S, int ST>
+__device__ __forceinline__ void rstages_split_from(float2 (&v)[M], float* lds, int stride,
+                                                   const float2* __restrict__ tw, int tid) {
+  rstage_compute<L, M, T, LINES, LFAST, S, ST>(v, tw, tid);
+  if constexpr (ST + 1 < RPlan<L, M>::stages) {
+    rexchange_split<L, M, T, LINES, LFAST, ST>(v, lds, stride, tid);
+    rstages_split_from<L, M, T, LINES, LFAST, S, ST + 1>(v, lds, stride, tw, tid);
+  }
+}
+
+template <int L, int M, int T, int LINES, bool LFAST, int S>
+__device__ __forceinline__ void rfft_split(float2 (&v)[M], float* lds, int stride, const float2* __restrict__ tw,
+                                           int tid) {
+  rstages_split_from<L, M, T, LINES, LFAST, S, 0>(v, lds, stride, tw, tid);
+}
+
 // G column groups of LINES lines each, time-multiplexing one LDS buffer of LINES lines: every
 // stage computes all groups, then exchanges group 0, group 1, ... through the same LDS (two
 // barriers per group and exchange). Lets a workgroup own G·LINES adjacent columns (wider row

@@ -638,6 +638,55 @@ static bool colfft_stream_LC(float2* A, int pa, float2* B, int pb, int nx, int r
   }
 }
 
+// Wide column tilings for long columns (L >= 1024): M points per thread, and optionally the
+// exchange split into real and imaginary halves (half the LDS: 16 columns of 2048 = 128-byte row
+// segments, or 8 columns of 4096, in one CU's LDS).
+template <int L, int C, int M, bool SPLIT>
+struct ColW {
+  static constexpr int T = C * L / M;
+  static constexpr int STR = line_stride(L, C);
+  static constexpr bool fits =
+      L >= 1024 && C * STR * (SPLIT ? 4 : 8) <= 160 * 1024 && T <= 1024 && T >= 64 && (C * L) % M == 0 &&
+      (M < 32 || T <= 512);  // 32 complex points per lane need more than the 128 VGPRs of a 1024-lane block
+};
+
+template <int L, int S, int C, int M, bool SPLIT>
+__global__ __launch_bounds__((ColW<L, C, M, SPLIT>::T), (SPLIT && M == 16 ? 2 : 1)) void k_colfft_w(float2* data, int nx, int planes, int rows_in,
+                                                                       float scale, const float2* __restrict__ tw) {
+  constexpr int T = ColW<L, C, M, SPLIT>::T, STR = ColW<L, C, M, SPLIT>::STR;
+  const int tid = threadIdx.x;
+  const int groups = nx / C;
+  const int lb = xcd_remap((int)blockIdx.x, groups * planes);
+  const int plane = lb / groups, x0 = (lb - plane * groups) * C;
+  float2* base = data + (int64_t)plane * nx * L + x0;
+  float2 v[M];
+  rload<L, M, T, C, true>(v, tid, [&](int c, int y) {
+    return y < rows_in ? base[(unsigned)(y * nx + c)] : make_float2(0.0f, 0.0f);
+  });
+  if constexpr (SPLIT) {
+    __shared__ float lds[C * STR];
+    rfft_split<L, M, T, C, true, S>(v, lds, STR, tw, tid);
+  } else {
+    __shared__ float2 lds[C * STR];
+    rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
+  }
+  rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+    base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
+  });
+}
+
+template <int L, int S, int C, int M, bool SPLIT>
+static bool colfft_w_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st) {
+  if constexpr (!ColW<L, C, M, SPLIT>::fits) {
+    return false;
+  } else {
+    if (nx % C) return false;
+    const unsigned g = (unsigned)(nx / C * planes);
+    k_colfft_w<L, S, C, M, SPLIT><<<g, ColW<L, C, M, SPLIT>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw);
+    return true;
+  }
+}
+
 template <int L, int S, int C>
 static bool colfft_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st) {
   if constexpr (!ColT<L, C>::fits) {
@@ -667,6 +716,15 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     case 68: return colfft_stream_LC<L, S, 4>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
     case 72: return colfft_stream_LC<L, S, 8>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
     case 80: return colfft_stream_LC<L, S, 16>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
+    // wide tilings (L >= 1024), 16 or 32 points per thread, split exchange unless noted:
+    // 41: 4x16, 42: 8x16, 43: 8x32, 44: 16x32, 45: 8x32 plain, 46: 4x32, 47: 2x32 plain
+    case 41: return colfft_w_LC<L, S, 4, 16, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 42: return colfft_w_LC<L, S, 8, 16, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 43: return colfft_w_LC<L, S, 8, 32, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 44: return colfft_w_LC<L, S, 16, 32, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 45: return colfft_w_LC<L, S, 8, 32, false>(data, planes, nx, rows_in, scale, tw, st);
+    case 46: return colfft_w_LC<L, S, 4, 32, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 47: return colfft_w_LC<L, S, 2, 32, false>(data, planes, nx, rows_in, scale, tw, st);
     default: return false;
   }
 }
@@ -995,71 +1053,101 @@ __global__ __launch_bounds__(256) void k_sat_carry(double* band, int W, int nb) 
     }
 }
 
-// Exclusive prefix of v over the 256 threads of the block (lds: 4 doubles).
-__device__ __forceinline__ double block_excl_scan(double v, double* lds) {
+// Exclusive prefix of G values per thread over the 256 threads of the block, each value its own
+// scan (lds: 4 · G doubles).
+template <int G>
+__device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double inc = v;
+  double inc[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) inc[g] = v[g];
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const double u = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += u;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const double u = __shfl_up(inc[g], o, 64);
+      if (lane >= o) inc[g] += u;
+    }
   }
-  if (lane == 63) lds[w] = inc;
+  if (lane == 63)
+#pragma unroll
+    for (int g = 0; g < G; ++g) lds[w * G + g] = inc[g];
   __syncthreads();
-  double off = 0.0;
-  for (int j = 0; j < w; ++j) off += lds[j];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    double off = 0.0;
+    for (int j = 0; j < w; ++j) off += lds[j * G + g];
+    v[g] = off + inc[g] - v[g];
+  }
   __syncthreads();   // lds is reused by the next scan
-  return off + inc - v;
 }
 
-// blockIdx.x band, blockIdx.y table; thread t owns columns [t·E, t·E + E).
-template <int E>
+// blockIdx.x band, blockIdx.y a group of G tables (one read of the canvas for all of them);
+// thread t owns columns [t·E, t·E + E).
+template <int E, int G>
 __global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canvas, int W, int H, SatArgs sa,
-                                                  const double* __restrict__ band, double* sat) {
-  __shared__ double lds[4];
-  const int b = blockIdx.x, k = blockIdx.y, nb = gridDim.x, pl = sa.plane[k];
+                                                  int n_tab, const double* __restrict__ band, double* sat) {
+  __shared__ double lds[4 * G];
+  const int b = blockIdx.x, k0 = blockIdx.y * G, nb = gridDim.x;
   const int xs = threadIdx.x * E;
-  const int64_t n = (int64_t)W * H, SW = W + 1;
-  double* S = sat + (int64_t)k * SW * (H + 1);
-  // column carry cc[e] = S(y, xs + e + 1): the row prefix of the earlier bands' column sums
-  double cc[E];
+  const int64_t n = (int64_t)W * H, SW = W + 1, SP = SW * (H + 1);
+  int pl[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) pl[g] = k0 + g < n_tab ? sa.plane[k0 + g] : 0;
+  // column carry cc[g][e] = S_g(y, xs + e + 1): the row prefix of the earlier bands' column sums
+  double cc[G][E];
   {
-    const double* crow = band + ((int64_t)k * nb + b) * W;
-    double tot = 0.0;
+    double tot[G];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      tot += xs + e < W ? crow[xs + e] : 0.0;
-      cc[e] = tot;
+    for (int g = 0; g < G; ++g) {
+      tot[g] = 0.0;
+      const double* crow = band + ((int64_t)min(k0 + g, n_tab - 1) * nb + b) * W;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        tot[g] += xs + e < W ? crow[xs + e] : 0.0;
+        cc[g][e] = tot[g];
+      }
     }
-    const double off = block_excl_scan(tot, lds);
+    block_excl_scan<G>(tot, lds);
 #pragma unroll
-    for (int e = 0; e < E; ++e) cc[e] += off;
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int e = 0; e < E; ++e) cc[g][e] += tot[g];
   }
   if (b == 0)
-    for (int x = threadIdx.x; x <= W; x += 256) S[x] = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < n_tab)
+        for (int x = threadIdx.x; x <= W; x += 256) sat[(k0 + g) * SP + x] = 0.0;
   const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
   for (int y = y0; y < y1; ++y) {
-    double v[E];
-    double tot = 0.0;
+    double v[G][E];
+    double tot[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) tot[g] = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      double d = 0.0;
-      if (xs + e < W) {
-        float sv[5];
-        canvas_values(canvas, n, (int64_t)y * W + xs + e, sv);
-        d = real_plane_d(pl, sv, canvas_inv0(sv));
+      float sv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      if (xs + e < W) canvas_values(canvas, n, (int64_t)y * W + xs + e, sv);
+      const double inv0 = canvas_inv0(sv);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        tot[g] += real_plane_d(pl[g], sv, inv0);
+        v[g][e] = tot[g];
       }
-      tot += d;
-      v[e] = tot;
     }
-    const double off = block_excl_scan(tot, lds);
-    double* row = S + (int64_t)(y + 1) * SW;
+    block_excl_scan<G>(tot, lds);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      cc[e] += off + v[e];
-      if (xs + e < W) row[xs + e + 1] = cc[e];
+    for (int g = 0; g < G; ++g) {
+      if (k0 + g >= n_tab) continue;
+      double* row = sat + (k0 + g) * SP + (int64_t)(y + 1) * SW;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        cc[g][e] += tot[g] + v[g][e];
+        if (xs + e < W) row[xs + e + 1] = cc[g][e];
+      }
+      if (threadIdx.x == 0) row[0] = 0.0;
     }
-    if (threadIdx.x == 0) row[0] = 0.0;
   }
 }
 
@@ -1071,9 +1159,10 @@ bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int 
   const int nb = sat_bands(H);
   k_sat_band<<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, sa, n_tab, band);
   k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 256, 0, st>>>(band, W, nb);
-  const dim3 g(nb, n_tab);
-  if (W <= 256 * 4) k_sat_rows<4><<<g, 256, 0, st>>>(canvas, W, H, sa, band, sat);
-  else k_sat_rows<8><<<g, 256, 0, st>>>(canvas, W, H, sa, band, sat);
+  constexpr int G = 3;   // tables per workgroup
+  const dim3 g(nb, (n_tab + G - 1) / G);
+  if (W <= 256 * 4) k_sat_rows<4, G><<<g, 256, 0, st>>>(canvas, W, H, sa, n_tab, band, sat);
+  else k_sat_rows<8, G><<<g, 256, 0, st>>>(canvas, W, H, sa, n_tab, band, sat);
   return true;
 }
 
