@@ -169,7 +169,7 @@ typedef struct {
 typedef struct {                 /* search.zig:145 Peak */
   double score;                  /* n in fixed pixels */
   uint32_t cand, it, is, ik, ia; /* candidate (theta-major), theta / sigma / shear / stretch index */
-  uint32_t _pad;
+  uint32_t _pad;                 /* 0 */
   double th, s;
   int32_t sx, sy;                /* canvas shift in cells */
   double H[9];                   /* corrected pose, moving -> fixed */

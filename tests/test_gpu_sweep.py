@@ -128,3 +128,30 @@ def test_sweep_nearest_grid_cell(reg_factory):
     b = Ht @ pts
     err = np.abs(a[:2] / a[2] - b[:2] / b[2]).max()
     assert err < 12.0, err
+
+
+def test_sweep_graph_replays_match_eager(reg_factory):
+    """The sweep's launches replayed from the captured HIP graph give the eager launches' peaks bit
+    for bit on every replay, with other GPU work and synchronisations between replays, and again
+    after the key changes (another batch size: a new capture)."""
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    mv, fx, _ = synth.rotscale_pair((128, 128), theta_deg=37.0, scale=1.12, seed=6)
+    kw = dict(half=False, sw_grid=64, sw_nth=12, sw_smin=0.85, sw_smax=1.15, sw_ns=3)
+    try:
+        for batch in (4, 7):
+            lib.zc_sweep_tune(batch)        # several batches per plane size: many launches per graph
+            eager = reg_factory(sweep_graph=False, **kw)
+            eager.set_images(mv, fx, preprocess=False)
+            _, pe, be = eager.sweep()
+            graph = reg_factory(sweep_graph=True, **kw)
+            graph.set_images(mv, fx, preprocess=False)
+            for r in range(4):
+                _, pg, bg = graph.sweep()
+                assert pg.tobytes() == pe.tobytes() and bg.tobytes() == be.tobytes(), (batch, r)
+                assert len(pg) > 0 and not pg["_pad"].any()
+                assert np.array_equal(graph.sweep_map(), eager.sweep_map()), (batch, r)
+                graph.shift_map(np.eye(3), exact=True, calibrated=False)   # eager launches between replays
+    finally:
+        lib.zc_sweep_tune(0)

@@ -101,6 +101,7 @@ Engine::~Engine() {
                   &gn_sums, &sat_band, &sat, &rs_fwd, &rs_inv, &rs_out, &rs_tab, &pre_buf, &pre_hist, &pre_cdf,
                   &pre_part, &pre_rgba})
     b->release();
+  sweep_gc.reset();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
   if (st) (void)hipStreamDestroy(st);
@@ -110,6 +111,46 @@ void Engine::release_side(Side& s) {
   for (DBuf* b : {&s.raw, &s.gray, &s.rank, &s.feat, &s.mag, &s.stack, &s.il}) b->release();
   s.has = false;
   s.il_ok = false;
+}
+
+void Engine::GraphCache::reset() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  exec = nullptr;
+  key.clear();
+  kernels = 0;
+}
+
+void Engine::run_captured(GraphCache& gc, std::vector<uint8_t>&& key, bool eager, const std::function<void()>& issue) {
+  if (eager) {
+    issue();
+    return;
+  }
+  if (!gc.exec || gc.key != key) {
+    gc.reset();
+    const uint32_t c0 = counts[0];
+    ZC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      issue();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      counts[0] = c0;
+      throw;
+    }
+    ZC_HIP(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    ZC_HIP(e);
+    gc.exec = ex;
+    gc.key = std::move(key);
+    gc.kernels = counts[0] - c0;
+    counts[0] = c0;   // capturing ran nothing
+  }
+  ZC_HIP(hipGraphLaunch(gc.exec, st));
+  counts[0] += gc.kernels;
+  prof.break_chain();
 }
 
 void Engine::sync() {

@@ -7,6 +7,7 @@
 #include <array>
 #include <atomic>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -285,6 +286,19 @@ class Engine {
 
   void fft2(float* data, int n, int planes, bool inverse);
   // every candidate's shift map reduced to its top local maxima (search.zig:300-494)
+  // A captured launch sequence, replayed while its key (every argument of every launch in it)
+  // is unchanged; `kernels` = the launches it holds (for the dispatch counter).
+  struct GraphCache {
+    std::vector<uint8_t> key;
+    hipGraphExec_t exec = nullptr;
+    uint32_t kernels = 0;
+    void reset();
+  };
+  GraphCache sweep_gc;
+  // Issues `issue` (kernel launches only on st: no copies, fills, syncs or allocations) through
+  // gc's graph: replayed when the key matches, else captured, instantiated and launched. Runs
+  // `issue` eagerly when `eager` (profiling needs per-launch events).
+  void run_captured(GraphCache& gc, std::vector<uint8_t>&& key, bool eager, const std::function<void()>& issue);
   SweepOut sweep(const SweepOpts& o);
   SweepOut sweep_sides(Side& A, Side& B, const SweepOpts& o);
   // batched pose evaluation (gn.cpp)

@@ -103,6 +103,7 @@ inline Mat3 sim_about(const std::array<double, 2>& c, double th, double sg) {
 struct SweepPeak {
   double score;
   uint32_t cand, it, is, ik, ia;
+  uint32_t pad;   // zero: the record crosses the C ABI byte for byte
   double th, s;
   int32_t sx, sy;
   Mat3 H;

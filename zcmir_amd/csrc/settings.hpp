@@ -124,6 +124,8 @@ static const SettingDef kSettings[] = {
     {"sw_rel", Kind::Bool, "false", nullptr},
     // not a reference option: the sweep at one canvas cell per fixed pixel (sw_grid ignored)
     {"sw_full", Kind::Bool, "false", nullptr},
+    // this library's: replay the sweep's launches from a captured HIP graph (eager while profiling)
+    {"sweep_graph", Kind::Bool, "true", nullptr},
     {"hho_auto", Kind::Bool, "true", nullptr},
     {"hho_n", Kind::U32, "100", nullptr},
     {"hho_t", Kind::U32, "20", nullptr},

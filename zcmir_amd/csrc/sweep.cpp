@@ -197,6 +197,33 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   ca.ridge = kRidge;
   ca.area = 1.0;
   const SweepPose* dposes = sw_poses.as<SweepPose>();
+  // everything the launches read from the host, resolved before a capture: twiddles and the
+  // column tilings (which may time the tilings on first use), then the key of the launch sequence
+  std::vector<uint8_t> key;
+  auto put = [&key](const auto& v) {
+    const auto* p = reinterpret_cast<const uint8_t*>(&v);
+    key.insert(key.end(), p, p + sizeof v);
+  };
+  for (const Group& gr : groups) {
+    put(gr);
+    put(twiddles((int)gr.nx));
+    put(twiddles((int)gr.ny));
+    put(col_lines((int)gr.ny));
+  }
+  put(plan);
+  put(n_peak);
+  put(ox);
+  put(oy);
+  put(cell);
+  put(grid);
+  put(Fx);
+  put(Fy);
+  put(ca);
+  for (const int v : {A.w, A.h, B.w, B.h}) put(v);
+  for (const void* p : {A.il.p, B.il.p, spec_a.p, spec_b.p, work.p, corr.p, sw_ns.p, sw_part.p, sw_peaks.p,
+                        sw_poses.p, sw_pose_fixed.p, sw_zero.p})
+    put(p);
+  auto issue = [&] {
   bool ok = true;
   for (const Group& gr : groups) {
     const uint32_t k0 = gr.k0, k1 = gr.k1, nx = gr.nx, ny = gr.ny, Bn = gr.bn;
@@ -238,11 +265,13 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
                                                    sw_zero.as<int>() + c0, st));
       LAUNCH("peaks", launch_peaks(sw_ns.as<float>(), (int)nx, (int)ny, b, (int)n_peak, sw_part.as<float2>(),
                                    sw_peaks.as<float2>() + (size_t)c0 * n_peak, st));
-      sweep_last_nx = nx;
-      sweep_last_ny = ny;
       if (!ok) throw std::runtime_error("UnsupportedFftLength");
     }
   }
+  };
+  run_captured(sweep_gc, std::move(key), prof.on || !set.b("sweep_graph"), issue);
+  sweep_last_nx = groups.back().nx;
+  sweep_last_ny = groups.back().ny;
   std::vector<float2> raw((size_t)nc * n_peak);
   ZC_HIP(hipMemcpyAsync(raw.data(), sw_peaks.p, raw.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
   prof.break_chain();
@@ -279,7 +308,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
       // the map peak displaces the moving image by +d; the correction is T(−d)
       const double dx = sx * cell, dy = sy * cell;
       const Mat3 H = mul3(translate(-dx, -dy), cd.H);
-      out.peaks.push_back({v, perm[slot], cd.it, cd.is, cd.ik, cd.ia, cd.th, cd.s, sx, sy, H});
+      out.peaks.push_back({v, perm[slot], cd.it, cd.is, cd.ik, cd.ia, 0, cd.th, cd.s, sx, sy, H});
     }
   }
   return out;
