@@ -562,9 +562,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   SatView sv{};
   if (rect) {
-    SatArgs sa{};
-    for (int k = 0; k < pi.n_sat; ++k) sa.plane[k] = pi.sat_plane[k];
-    LAUNCH("sat_build", ok &= launch_sat_build(canvas_a.as<float>(), W, Hh, sa, pi.n_sat, sat_band.as<double>(), sat.as<double>(), st));
+    LAUNCH("sat_build", ok &= launch_sat_build(plan, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), st));
     sv.sat = sat.as<double>();
     sv.W = W;
     sv.H = Hh;

@@ -988,6 +988,8 @@ constexpr int kSatBand = 8;   // rows per band
 
 // The real planes of a canvas stack sample in FP64: w, w f_i, and w f_i f_j = s_i s_j / s_0
 // (0 where s_0 <= 1e-6), the last with one reciprocal per sample.
+// pl is a compile-time constant at every call (the plan's table → plane map), so every index
+// below folds and the sample stays in registers.
 __device__ __forceinline__ double real_plane_d(int pl, const float* s, double inv0) {
   if (pl < 5) return (double)s[pl];
   if (s[0] <= 1e-6f) return 0.0;
@@ -1006,8 +1008,10 @@ __device__ __forceinline__ void canvas_values(const float* __restrict__ canvas, 
 
 // blockIdx.x band, blockIdx.y a chunk of 256 columns: band[(k · nb + b) · W + x] = column sums of
 // table k's plane over the band's rows, for every table.
-__global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canvas, int W, int H, SatArgs sa,
-                                                  int n_tab, double* band) {
+template <int PID>
+__global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canvas, int W, int H, double* band) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int n_tab = P.n_sat;
   const int b = blockIdx.x, nb = gridDim.x;
   const int x = blockIdx.y * 256 + threadIdx.x;
   if (x >= W) return;
@@ -1016,13 +1020,20 @@ __global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canv
   double acc[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-  for (int y = y0; y < y1; ++y) {
-    float sv[5];
-    canvas_values(canvas, n, (int64_t)y * W + x, sv);
-    const double inv0 = canvas_inv0(sv);
+  float sv[kSatBand][5];
 #pragma unroll
-    for (int k = 0; k < 15; ++k)
-      if (k < n_tab) acc[k] += real_plane_d(sa.plane[k], sv, inv0);
+  for (int i = 0; i < kSatBand; ++i) {   // every row's loads in flight together
+    const int y = min(y0 + i, H - 1);
+    canvas_values(canvas, n, (int64_t)y * W + x, sv[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < kSatBand; ++i) {
+    if (y0 + i < y1) {
+      const double inv0 = canvas_inv0(sv[i]);
+#pragma unroll
+      for (int k = 0; k < 15; ++k)
+        if (k < n_tab) acc[k] += real_plane_d(P.sat_plane[k], sv[i], inv0);
+    }
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k)
@@ -1030,13 +1041,14 @@ __global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canv
 }
 
 // In place: band sums → exclusive prefix over bands, per (table, column). blockIdx.x: table ·
-// column chunks of 64; the block's 4 waves take a quarter of the bands each.
-__global__ __launch_bounds__(256) void k_sat_carry(double* band, int W, int nb) {
-  __shared__ double seg[4][64];
+// column chunks of 64; the block's 16 waves take a sixteenth of the bands each.
+constexpr int kCarryWaves = 16;
+__global__ __launch_bounds__(64 * kCarryWaves) void k_sat_carry(double* band, int W, int nb) {
+  __shared__ double seg[kCarryWaves][64];
   const int chunks = (W + 63) / 64;
   const int k = blockIdx.x / chunks, x = (blockIdx.x - k * chunks) * 64 + (threadIdx.x & 63);
   const int q = threadIdx.x >> 6;
-  const int per = (nb + 3) / 4, b0 = q * per, b1 = min(nb, b0 + per);
+  const int per = (nb + kCarryWaves - 1) / kCarryWaves, b0 = q * per, b1 = min(nb, b0 + per);
   double* col = band + (int64_t)k * nb * W + x;
   double tot = 0.0;
   if (x < W)
@@ -1083,17 +1095,20 @@ __device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
 }
 
 // blockIdx.x band, blockIdx.y a group of G tables (one read of the canvas for all of them);
-// thread t owns columns [t·E, t·E + E).
-template <int E, int G>
-__global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canvas, int W, int H, SatArgs sa,
-                                                  int n_tab, const double* __restrict__ band, double* sat) {
-  __shared__ double lds[4 * G];
-  const int b = blockIdx.x, k0 = blockIdx.y * G, nb = gridDim.x;
+// thread t owns columns [t·E, t·E + E) for the scans; each finished row goes out through LDS so
+// that lane i stores column x + i (whole cache lines per store instruction).
+template <int PID, int E, int G, int K0>
+__device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, int W, int H,
+                                              const double* __restrict__ band, double* sat, double* lds,
+                                              double (*stage)[256 * E]) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int n_tab = P.n_sat, k0 = K0;
+  const int b = blockIdx.x, nb = gridDim.x;
   const int xs = threadIdx.x * E;
   const int64_t n = (int64_t)W * H, SW = W + 1, SP = SW * (H + 1);
-  int pl[G];
-#pragma unroll
-  for (int g = 0; g < G; ++g) pl[g] = k0 + g < n_tab ? sa.plane[k0 + g] : 0;
+  constexpr int pl[G] = {P.sat_plane[k0 < n_tab ? k0 : 0], P.sat_plane[k0 + 1 < n_tab ? k0 + 1 : 0],
+                         P.sat_plane[k0 + 2 < n_tab ? k0 + 2 : 0]};
+  static_assert(G == 3, "three tables per workgroup");
   // column carry cc[g][e] = S_g(y, xs + e + 1): the row prefix of the earlier bands' column sums
   double cc[G][E];
   {
@@ -1120,15 +1135,32 @@ __global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canv
       if (k0 + g < n_tab)
         for (int x = threadIdx.x; x <= W; x += 256) sat[(k0 + g) * SP + x] = 0.0;
   const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
+  // the canvas values of the thread's columns, one row ahead of the scans (E <= 4: the registers
+  // of a second row fit)
+  constexpr bool AHEAD = E <= 4;
+  float nxt[E][5];
+#define ZC_SAT_LOAD_ROW(YY)                                                                         \
+  _Pragma("unroll") for (int e = 0; e < E; ++e) {                                                    \
+    _Pragma("unroll") for (int p = 0; p < 5; ++p) nxt[e][p] = 0.0f;                                  \
+    if ((YY) < y1 && xs + e < W)                                                                     \
+      _Pragma("unroll") for (int p = 0; p < 5; ++p) nxt[e][p] = canvas[p * n + (int64_t)(YY) * W + xs + e]; \
+  }
+  if (AHEAD) ZC_SAT_LOAD_ROW(y0)
   for (int y = y0; y < y1; ++y) {
+    if (!AHEAD) ZC_SAT_LOAD_ROW(y)
+    float cur[E][5];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+#pragma unroll
+      for (int p = 0; p < 5; ++p) cur[e][p] = nxt[e][p];
+    if (AHEAD) ZC_SAT_LOAD_ROW(y + 1)
     double v[G][E];
     double tot[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) tot[g] = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      float sv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-      if (xs + e < W) canvas_values(canvas, n, (int64_t)y * W + xs + e, sv);
+      const float* sv = cur[e];
       const double inv0 = canvas_inv0(sv);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -1136,34 +1168,65 @@ __global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canv
         v[g][e] = tot[g];
       }
     }
-    block_excl_scan<G>(tot, lds);
+    block_excl_scan<G>(tot, lds);   // its barriers also order the previous row's stage reads
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        cc[g][e] += tot[g] + v[g][e];
+        stage[g][xs + e] = cc[g][e];
+      }
+    __syncthreads();
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (k0 + g >= n_tab) continue;
       double* row = sat + (k0 + g) * SP + (int64_t)(y + 1) * SW;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        cc[g][e] += tot[g] + v[g][e];
-        if (xs + e < W) row[xs + e + 1] = cc[g][e];
-      }
+      for (int x = threadIdx.x; x < W; x += 256) row[x + 1] = stage[g][x];
       if (threadIdx.x == 0) row[0] = 0.0;
     }
+  }
+#undef ZC_SAT_LOAD_ROW
+}
+
+// blockIdx.y: the group of G tables (tables G·y .. G·y + G − 1 of the plan)
+template <int PID, int E, int G>
+__global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canvas, int W, int H,
+                                                  const double* __restrict__ band, double* sat) {
+  __shared__ double lds[4 * G];
+  __shared__ double stage[G][256 * E];
+  static_assert(kPlans[PID].n_sat <= 5 * G, "at most five groups");
+  switch (blockIdx.y) {
+    case 0: sat_rows_body<PID, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
+    case 1: sat_rows_body<PID, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
+    case 2: sat_rows_body<PID, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
+    case 3: sat_rows_body<PID, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
+    default: sat_rows_body<PID, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
   }
 }
 
 int sat_bands(int H) { return (H + kSatBand - 1) / kSatBand; }
 
-bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int n_tab, double* band, double* sat,
-                      hipStream_t st) {
-  if (W < 1 || H < 1 || W > 256 * 8 || n_tab < 1 || n_tab > 15) return false;
+template <int PID>
+static void sat_build_P(const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+  constexpr int n_tab = kPlans[PID].n_sat;
   const int nb = sat_bands(H);
-  k_sat_band<<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, sa, n_tab, band);
-  k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 256, 0, st>>>(band, W, nb);
+  k_sat_band<PID><<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, band);
+  k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 64 * kCarryWaves, 0, st>>>(band, W, nb);
   constexpr int G = 3;   // tables per workgroup
   const dim3 g(nb, (n_tab + G - 1) / G);
-  if (W <= 256 * 4) k_sat_rows<4, G><<<g, 256, 0, st>>>(canvas, W, H, sa, n_tab, band, sat);
-  else k_sat_rows<8, G><<<g, 256, 0, st>>>(canvas, W, H, sa, n_tab, band, sat);
-  return true;
+  if (W <= 256 * 4) k_sat_rows<PID, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
+  else k_sat_rows<PID, 8, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
+}
+
+bool launch_sat_build(int plan, const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+  if (W < 1 || H < 1 || W > 256 * 8) return false;
+  switch (plan) {
+    case 4: sat_build_P<4>(canvas, W, H, band, sat, st); return true;
+    case 5: sat_build_P<5>(canvas, W, H, band, sat, st); return true;
+    case 6: sat_build_P<6>(canvas, W, H, band, sat, st); return true;
+    case 7: sat_build_P<7>(canvas, W, H, band, sat, st); return true;
+    default: return false;
+  }
 }
 
 // Table offsets of the rectangle R + (dx, dy) clipped to the canvas for lag index t.

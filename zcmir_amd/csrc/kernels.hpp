@@ -58,14 +58,11 @@ struct CombineArgs {
   double area, ridge;
 };
 int combine_blocks(int64_t nn);
-// Summed-area tables of the rectangle-sum planes (rectangle plans): plane[k] for table k.
-struct SatArgs {
-  int plane[15];
-};
+// Summed-area tables of the rectangle-sum planes of a rectangle plan (4..7): table k of plane
+// kPlans[plan].sat_plane[k].
 int sat_bands(int H);
-// canvas: 5 planes W x H; band: n_tab · sat_bands(H) · W doubles; sat: n_tab · (W+1)(H+1) doubles
-bool launch_sat_build(const float* canvas, int W, int H, const SatArgs& sa, int n_tab, double* band, double* sat,
-                      hipStream_t st);
+// canvas: 5 planes W x H; band: n_sat · sat_bands(H) · W doubles; sat: n_sat · (W+1)(H+1) doubles
+bool launch_sat_build(int plan, const float* canvas, int W, int H, double* band, double* sat, hipStream_t st);
 // The tables and the fixed rectangle [bx0, bx1) x [by0, by1) in canvas cells; nx = 1 << lx.
 struct SatView {
   const double* sat;
