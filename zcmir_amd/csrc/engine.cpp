@@ -537,8 +537,11 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   const size_t cn = (size_t)W * Hh;
   canvas_a.ensure(cn * 20);
   canvas_b.ensure(cn * 20);
-  spec_a.ensure((size_t)N * 8 * pi.n_cplx);
-  spec_b.ensure((size_t)N * 8 * pi.b_n_cplx);
+  // both sides' spectra in one buffer (side B's planes right after side A's): one forward column
+  // pass over all of them
+  spec_a.ensure((size_t)N * 8 * (pi.n_cplx + pi.b_n_cplx));
+  float2* const spa = spec_a.as<float2>();
+  float2* const spb = spa + N * pi.n_cplx;
   if (rect) {
     sat_band.ensure((size_t)pi.n_sat * sat_bands(Hh) * W * 8);
     sat.ensure((size_t)pi.n_sat * (W + 1) * (Hh + 1) * 8);
@@ -556,10 +559,9 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
   LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
   bool ok = true;
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spec_a.as<float2>(), twiddles(nx), st));
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spec_b.as<float2>(), twiddles(nx), st));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spec_b.as<float2>(), pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   SatView sv{};
   if (rect) {
     LAUNCH("sat_build", ok &= launch_sat_build(plan, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), st));
@@ -576,7 +578,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     sv.by0 = std::max(0, -m.fr.oy);
     sv.by1 = std::min(Hh, -m.fr.oy + B.h);
   }
-  LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), nx, ny, st));
+  LAUNCH("cmul", launch_cmul(plan, spa, spb, work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
   LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
