@@ -237,6 +237,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a copy kernel of this many MiB and exit")
     ap.add_argument("--fft-cols", type=int, default=0, help="columns per workgroup of the column FFT (0: default)")
+    ap.add_argument("--fft-rows", type=int, default=0, help="tiling of the inverse row FFT, 0-3 (0: default)")
     ap.add_argument("--sweep-steps", type=int, default=10, help="timed sweeps on the 128² canvas (0: skip)")
     ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
     ap.add_argument("--prior-pairs", type=int, default=6, help="time-to-pose pairs, learned vs full (0: skip)")
@@ -257,8 +258,8 @@ def main():
     from zcmir_amd import synth
 
     three = a.corr == 28
-    if a.fft_cols:
-        zcmir_amd._load().zc_fft_tune(0, a.fft_cols)
+    if a.fft_cols or a.fft_rows:
+        zcmir_amd._load().zc_fft_tune(a.fft_rows, a.fft_cols)
     size, shift = (1024, 1024), (113, -57)
     mv, fx, _ = synth.shift_pair(size, shift, seed=11, quantize=True)
     reg = zcmir_amd.Registrar(half=False, symmetric=False, smi_features="three" if three else "laplacian")

@@ -143,8 +143,11 @@ size_t zc_profile_report(void* e, char* buf, size_t cap);
 void zc_gpu_counts(void* e, uint32_t out[4]);
 /* exports.zig:497 */
 void zc_result_sizes(uint32_t out[4]);
-/* exports.zig:95: lines per workgroup of the batched FFTs for rows / columns (0: default), for
- * FFTs run from now on; process-wide */
+/* exports.zig:95: tilings of the batched FFTs for rows / columns (0: default), for FFTs run from
+ * now on; process-wide. rows: the inverse row pass over lines of >= 1024 points (0: two lines
+ * per workgroup up to 2048, 16 points per lane; 1: 32 points per lane; 2: 8; 3: one line per
+ * workgroup).
+ * cols: the column-FFT tiling code */
 void zc_fft_tune(uint32_t rows, uint32_t cols);
 /* this library's: the column-FFT tiling used for column length L (zc_fft_tune's value, else the
  * autotuned one when the setting fft_autotune is on, timing the tilings now if not yet done,

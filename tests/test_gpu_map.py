@@ -204,3 +204,28 @@ def test_autotuned_tiling_leaves_maps_unchanged(reg_factory):
             assert reg.fft_tiling(512) in (0, 1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47)
         maps.append(reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=1024).map)
     assert np.array_equal(maps[0].view(np.uint32), maps[1].view(np.uint32))
+
+
+def test_row_tilings_of_the_inverse_pass(reg_factory):
+    """The inverse row pass's tilings (zc_fft_tune rows): those with the default's stages (16·16·8
+    for 2048) give the default map bit for bit; 8 points per lane (8·8·8·4) within float32
+    rounding, with the same peak."""
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    mv, fx, _ = _pair((1024, 1024), (113, -57), seed=11, quantize=True)
+    maps = {}
+    try:
+        for rows in (0, 1, 2, 3):
+            lib.zc_fft_tune(rows, 0)
+            reg = reg_factory(half=False)
+            reg.set_images(mv, fx, preprocess=False)
+            sm = reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=2048)
+            assert (sm.dx, sm.dy) == (113.0, -57.0), rows
+            maps[rows] = sm.map
+            reg.close()
+    finally:
+        lib.zc_fft_tune(0, 0)
+    for rows in (1, 3):
+        assert np.array_equal(maps[rows].view(np.uint32), maps[0].view(np.uint32)), rows
+    assert np.abs(maps[2] - maps[0]).max() <= 1e-5 * np.abs(maps[0]).max()

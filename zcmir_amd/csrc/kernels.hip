@@ -788,11 +788,24 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
 // ── row IFFT + extract ────────────────────────────────────────────────────────────────────
 // Inverse row FFT of complex output q (rows of L) → correlation planes 2q (re) and 2q+1 (im).
 // blockIdx.y = b · n_out + q for batch entry b (its planes at b · n_corr).
-template <int L, int PID>
-__global__ __launch_bounds__(RowT<L>::T) void k_rowifft_extract(const float2* __restrict__ work, float* corr, int ny,
-                                                                const float2* __restrict__ tw) {
+// Row tilings of the inverse row pass (zc_fft_tune's rows value) for lines of >= 1024 points:
+// 0 (default) 16 points per lane, two lines per workgroup at 1024 and 2048 (0.237 -> 0.224 ms
+// for the 15 planes of a 2048² map), one at 4096 (two: 4 % slower); 1 32 points per lane; 2 8
+// points per lane; 3 16 points per lane, one line per workgroup. Shorter lines always take
+// RowT's tiling.
+template <int L, int V>
+struct RowX {
+  static constexpr int M = V == 1 ? 32 : V == 2 ? 8 : 16;
+  static constexpr int LINES = RowT<L>::LINES * (V == 0 && L >= 1024 && L <= 2048 ? 2 : 1);
+  static constexpr int T = LINES * L / M;
+  static constexpr int STR = RowT<L>::STR;
+};
+
+template <int L, int PID, int V>
+__global__ __launch_bounds__((RowX<L, V>::T)) void k_rowifft_extract(const float2* __restrict__ work, float* corr,
+                                                                    int ny, const float2* __restrict__ tw) {
   constexpr CorrPlan P = kPlans[PID];
-  constexpr int T = RowT<L>::T, M = RowT<L>::M, LINES = RowT<L>::LINES, STR = RowT<L>::STR;
+  constexpr int T = RowX<L, V>::T, M = RowX<L, V>::M, LINES = RowX<L, V>::LINES, STR = RowX<L, V>::STR;
   __shared__ float2 lds[LINES * STR];
   const int tid = threadIdx.x, g = blockIdx.y;
   const int q = g % P.n_out, b = g / P.n_out;
@@ -2030,19 +2043,33 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
   }
 }
 
+template <int L, int V>
+static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
+  using X = RowX<L, V>;
+  dim3 g((ny + X::LINES - 1) / X::LINES, kPlans[plan].n_out * batch);
+  switch (plan) {
+    case 0: k_rowifft_extract<L, 0, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 1: k_rowifft_extract<L, 1, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 2: k_rowifft_extract<L, 2, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 3: k_rowifft_extract<L, 3, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 4: k_rowifft_extract<L, 4, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 5: k_rowifft_extract<L, 5, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 6: k_rowifft_extract<L, 6, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    default: k_rowifft_extract<L, 7, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+  }
+}
+
 template <int L>
 static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
-  dim3 g((ny + RowT<L>::LINES - 1) / RowT<L>::LINES, kPlans[plan].n_out * batch);
-  switch (plan) {
-    case 0: k_rowifft_extract<L, 0><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    case 1: k_rowifft_extract<L, 1><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    case 2: k_rowifft_extract<L, 2><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    case 3: k_rowifft_extract<L, 3><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    case 4: k_rowifft_extract<L, 4><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    case 5: k_rowifft_extract<L, 5><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    case 6: k_rowifft_extract<L, 6><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
-    default: k_rowifft_extract<L, 7><<<g, RowT<L>::T, 0, st>>>(work, corr, ny, tw); break;
+  if constexpr (L >= 1024) {
+    switch (FftRowTiling::v) {
+      case 1: rowifft_LV<L, 1>(plan, work, corr, ny, tw, st, batch); return;
+      case 2: rowifft_LV<L, 2>(plan, work, corr, ny, tw, st, batch); return;
+      case 3: rowifft_LV<L, 3>(plan, work, corr, ny, tw, st, batch); return;
+      default: break;
+    }
   }
+  rowifft_LV<L, 0>(plan, work, corr, ny, tw, st, batch);
 }
 
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,

@@ -2,9 +2,17 @@
 // pointers only. Sizes are validated by the engine before any launch.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
 
 namespace zc {
+
+// Tiling of the inverse row pass for lines of >= 1024 points (zc_fft_tune's rows value): 0 16
+// points per lane with two lines per workgroup (one at 4096), 1 32 points per lane, 2 8, 3 16
+// with one line per workgroup; process-wide.
+struct FftRowTiling {
+  static inline std::atomic<int> v{0};
+};
 
 struct WhitenArgs {
   double X[16];   // lower-triangular L^-1, row-major
