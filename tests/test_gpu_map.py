@@ -178,7 +178,7 @@ def test_fft2_every_column_tiling(reg_factory, n):
     reg = reg_factory()
     ran = 0
     try:
-        for cols in (1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47):
+        for cols in (1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47, 48, 49):
             lib.zc_fft_tune(0, cols)
             try:
                 y = reg.fft2(x.copy())
@@ -201,7 +201,7 @@ def test_autotuned_tiling_leaves_maps_unchanged(reg_factory):
         reg = reg_factory(half=False, fft_autotune=tune)
         reg.set_images(mv, fx, preprocess=False)
         if tune:
-            assert reg.fft_tiling(512) in (0, 1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47)
+            assert reg.fft_tiling(512) in (0, 1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47, 48, 49)
         maps.append(reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=1024).map)
     assert np.array_equal(maps[0].view(np.uint32), maps[1].view(np.uint32))
 

@@ -223,7 +223,7 @@ int Engine::autotune_cols(int L) {
   };
   double t_best = time(0);
   int c_best = 0;
-  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47}) {
+  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47, 48, 49}) {
     const double t = time(lines);
     if (t < 0.97 * t_best) {
       t_best = t;

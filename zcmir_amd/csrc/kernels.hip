@@ -647,7 +647,8 @@ struct ColW {
   static constexpr int STR = line_stride(L, C);
   static constexpr bool fits =
       L >= 1024 && C * STR * (SPLIT ? 4 : 8) <= 160 * 1024 && T <= 1024 && T >= 64 && (C * L) % M == 0 &&
-      (M < 32 || T <= 512);  // 32 complex points per lane need more than the 128 VGPRs of a 1024-lane block
+      (M < 32 || T <= 512) &&   // 32 complex points per lane need more than the 128 VGPRs of a 1024-lane block
+      (M < 64 || T == 512);     // 64 points: 128 VGPRs of data, two waves per SIMD
 };
 
 template <int L, int S, int C, int M, bool SPLIT>
@@ -717,7 +718,8 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     case 72: return colfft_stream_LC<L, S, 8>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
     case 80: return colfft_stream_LC<L, S, 16>(data, planes, nullptr, 0, nx, rows_in, scale, tw, st);
     // wide tilings (L >= 1024), 16 or 32 points per thread, split exchange unless noted:
-    // 41: 4x16, 42: 8x16, 43: 8x32, 44: 16x32, 45: 8x32 plain, 46: 4x32, 47: 2x32 plain
+    // 41: 4x16, 42: 8x16, 43: 8x32, 44: 16x32, 45: 8x32 plain, 46: 4x32, 47: 2x32 plain,
+    // 48: 16x64, 49: 8x64
     case 41: return colfft_w_LC<L, S, 4, 16, true>(data, planes, nx, rows_in, scale, tw, st);
     case 42: return colfft_w_LC<L, S, 8, 16, true>(data, planes, nx, rows_in, scale, tw, st);
     case 43: return colfft_w_LC<L, S, 8, 32, true>(data, planes, nx, rows_in, scale, tw, st);
@@ -725,6 +727,8 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     case 45: return colfft_w_LC<L, S, 8, 32, false>(data, planes, nx, rows_in, scale, tw, st);
     case 46: return colfft_w_LC<L, S, 4, 32, true>(data, planes, nx, rows_in, scale, tw, st);
     case 47: return colfft_w_LC<L, S, 2, 32, false>(data, planes, nx, rows_in, scale, tw, st);
+    case 48: return colfft_w_LC<L, S, 16, 64, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 49: return colfft_w_LC<L, S, 8, 64, true>(data, planes, nx, rows_in, scale, tw, st);
     default: return false;
   }
 }
