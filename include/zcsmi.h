@@ -145,8 +145,8 @@ void zc_gpu_counts(void* e, uint32_t out[4]);
 void zc_result_sizes(uint32_t out[4]);
 /* exports.zig:95: tilings of the batched FFTs for rows / columns (0: default), for FFTs run from
  * now on; process-wide. rows: the inverse row pass over lines of >= 1024 points (0: two lines
- * per workgroup up to 2048, 16 points per lane; 1: 32 points per lane; 2: 8; 3: one line per
- * workgroup).
+ * per workgroup at 1024, 32 points per lane from 2048; 1: 32 points per lane; 2: 8; 3: 16, one
+ * line per workgroup).
  * cols: the column-FFT tiling code */
 void zc_fft_tune(uint32_t rows, uint32_t cols);
 /* this library's: the column-FFT tiling used for column length L (zc_fft_tune's value, else the

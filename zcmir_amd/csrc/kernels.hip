@@ -793,14 +793,17 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
 // Inverse row FFT of complex output q (rows of L) → correlation planes 2q (re) and 2q+1 (im).
 // blockIdx.y = b · n_out + q for batch entry b (its planes at b · n_corr).
 // Row tilings of the inverse row pass (zc_fft_tune's rows value) for lines of >= 1024 points:
-// 0 (default) 16 points per lane, two lines per workgroup at 1024 and 2048 (0.237 -> 0.224 ms
-// for the 15 planes of a 2048² map), one at 4096 (two: 4 % slower); 1 32 points per lane; 2 8
-// points per lane; 3 16 points per lane, one line per workgroup. Shorter lines always take
-// RowT's tiling.
+// 0 (default) two 1024-point lines per workgroup at 16 points per lane, 32 points per lane from
+// 2048 (measured: 0.237 -> 0.225 ms for the 15 planes of a 2048² map, 1.54 -> 1.33 ms for a
+// sweep batch's 4096-point planes); 1 32 points per lane; 2 8 points per lane; 3 16 points per
+// lane, one line per workgroup. Shorter lines always take RowT's tiling. 2048 and 4096 split
+// into radix-16/8 stages the same way at 16 and 32 points per lane, so those tilings give the
+// same results bit for bit.
 template <int L, int V>
 struct RowX {
-  static constexpr int M = V == 1 ? 32 : V == 2 ? 8 : 16;
-  static constexpr int LINES = RowT<L>::LINES * (V == 0 && L >= 1024 && L <= 2048 ? 2 : 1);
+  static constexpr bool WIDE = V == 1 || (V == 0 && L >= 2048);
+  static constexpr int M = WIDE ? 32 : V == 2 ? 8 : 16;
+  static constexpr int LINES = RowT<L>::LINES * (V == 0 && L == 1024 ? 2 : 1);
   static constexpr int T = LINES * L / M;
   static constexpr int STR = RowT<L>::STR;
 };

@@ -7,8 +7,8 @@
 
 namespace zc {
 
-// Tiling of the inverse row pass for lines of >= 1024 points (zc_fft_tune's rows value): 0 16
-// points per lane with two lines per workgroup (one at 4096), 1 32 points per lane, 2 8, 3 16
+// Tiling of the inverse row pass for lines of >= 1024 points (zc_fft_tune's rows value): 0 two
+// lines per workgroup at 1024, 32 points per lane from 2048; 1 32 points per lane; 2 8; 3 16
 // with one line per workgroup; process-wide.
 struct FftRowTiling {
   static inline std::atomic<int> v{0};
