@@ -144,9 +144,9 @@ void zc_gpu_counts(void* e, uint32_t out[4]);
 /* exports.zig:497 */
 void zc_result_sizes(uint32_t out[4]);
 /* exports.zig:95: tilings of the batched FFTs for rows / columns (0: default), for FFTs run from
- * now on; process-wide. rows: the inverse row pass over lines of >= 1024 points (0: two lines
- * per workgroup at 1024, 32 points per lane from 2048; 1: 32 points per lane; 2: 8; 3: 16, one
- * line per workgroup).
+ * now on; process-wide. rows: the inverse row pass over lines of >= 256 points (0: 8 points per
+ * lane at 256, two lines per workgroup at 1024, 32 points per lane from 2048; 1: 32 points per
+ * lane; 2: 8; 3: 16; 4: 16 with twice the lines per workgroup).
  * cols: the column-FFT tiling code */
 void zc_fft_tune(uint32_t rows, uint32_t cols);
 /* this library's: the column-FFT tiling used for column length L (zc_fft_tune's value, else the

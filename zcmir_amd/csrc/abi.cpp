@@ -580,7 +580,7 @@ void zc_tune_parts(uint32_t, uint32_t, uint32_t, uint32_t) {}
 // FFTs run from now on (an unsupported value makes those FFTs fail with UnsupportedFftLength)
 void zc_fft_tune(uint32_t rows, uint32_t cols) {
   zc::FftTune::rows = (int)rows;
-  zc::FftRowTiling::v = rows <= 3 ? (int)rows : 0;
+  zc::FftRowTiling::v = rows <= 4 ? (int)rows : 0;
   zc::FftTune::cols = (int)cols;
 }
 void zc_fft_tune_wg(uint32_t, uint32_t, uint32_t) {}

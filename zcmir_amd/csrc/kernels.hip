@@ -792,18 +792,23 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
 // ── row IFFT + extract ────────────────────────────────────────────────────────────────────
 // Inverse row FFT of complex output q (rows of L) → correlation planes 2q (re) and 2q+1 (im).
 // blockIdx.y = b · n_out + q for batch entry b (its planes at b · n_corr).
-// Row tilings of the inverse row pass (zc_fft_tune's rows value) for lines of >= 1024 points:
-// 0 (default) two 1024-point lines per workgroup at 16 points per lane, 32 points per lane from
-// 2048 (measured: 0.237 -> 0.225 ms for the 15 planes of a 2048² map, 1.54 -> 1.33 ms for a
-// sweep batch's 4096-point planes); 1 32 points per lane; 2 8 points per lane; 3 16 points per
-// lane, one line per workgroup. Shorter lines always take RowT's tiling. 2048 and 4096 split
-// into radix-16/8 stages the same way at 16 and 32 points per lane, so those tilings give the
-// same results bit for bit.
+// Row tilings of the inverse row pass (zc_fft_tune's rows value) for lines of >= 256 points:
+// 0 (default) 8 points per lane at 256 (0.287 -> 0.270 ms for a 128²-canvas sweep batch), two
+// 1024-point lines per workgroup at 16 points per lane, 32 points per lane from 2048 (0.237 ->
+// 0.225 ms for the 15 planes of a 2048² map, 1.54 -> 1.33 ms for a sweep batch's 4096-point
+// planes), RowT's tiling otherwise; 1 32 points per lane; 2 8 points
+// per lane; 3 16 points per lane with RowT's lines per workgroup; 4 16 points per lane, twice
+// RowT's lines. Every workgroup holds at least one wave. Shorter lines always take RowT's
+// tiling. Lines whose radix-16/8 stages split the same way at 16 and 32 points per lane (2048,
+// 4096) give the same results bit for bit at either.
 template <int L, int V>
 struct RowX {
   static constexpr bool WIDE = V == 1 || (V == 0 && L >= 2048);
-  static constexpr int M = WIDE ? 32 : V == 2 ? 8 : 16;
-  static constexpr int LINES = RowT<L>::LINES * (V == 0 && L == 1024 ? 2 : 1);
+  static constexpr int M = WIDE ? 32 : (V == 2 || (V == 0 && L == 256)) ? 8 : 16;
+  static constexpr int TWO = (V == 4 || (V == 0 && L == 1024)) ? 2 : 1;
+  static constexpr int MIN_LINES = (64 * M + L - 1) / L;
+  static constexpr int LINES =
+      RowT<L>::LINES * TWO > MIN_LINES ? RowT<L>::LINES * TWO : MIN_LINES;
   static constexpr int T = LINES * L / M;
   static constexpr int STR = RowT<L>::STR;
 };
@@ -2068,11 +2073,12 @@ static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const 
 
 template <int L>
 static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
-  if constexpr (L >= 1024) {
+  if constexpr (L >= 256) {
     switch (FftRowTiling::v) {
       case 1: rowifft_LV<L, 1>(plan, work, corr, ny, tw, st, batch); return;
       case 2: rowifft_LV<L, 2>(plan, work, corr, ny, tw, st, batch); return;
       case 3: rowifft_LV<L, 3>(plan, work, corr, ny, tw, st, batch); return;
+      case 4: rowifft_LV<L, 4>(plan, work, corr, ny, tw, st, batch); return;
       default: break;
     }
   }

@@ -7,9 +7,9 @@
 
 namespace zc {
 
-// Tiling of the inverse row pass for lines of >= 1024 points (zc_fft_tune's rows value): 0 two
-// lines per workgroup at 1024, 32 points per lane from 2048; 1 32 points per lane; 2 8; 3 16
-// with one line per workgroup; process-wide.
+// Tiling of the inverse row pass for lines of >= 256 points (zc_fft_tune's rows value): 0 8
+// points per lane at 256, two lines per workgroup at 1024, 32 points per lane from 2048; 1 32
+// points per lane; 2 8; 3 16; 4 16 with twice the lines per workgroup; process-wide.
 struct FftRowTiling {
   static inline std::atomic<int> v{0};
 };
