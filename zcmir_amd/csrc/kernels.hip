@@ -66,7 +66,11 @@ __global__ __launch_bounds__(256) void k_maxabs(const float* __restrict__ x, int
     m = max(m, __float_as_uint(fabsf(x[i])));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_down((int)m, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+  // one atomic per workgroup (atomics on one address serialize)
+  __shared__ uint32_t wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, max(max(wm[0], wm[1]), max(wm[2], wm[3])));
 }
 
 // Exact fixed-point sums: term = trunc(v · 2^K), v = x or f32(x - mean)^2 (exact in f64),
@@ -92,9 +96,18 @@ __global__ __launch_bounds__(256) void k_fixsum(const float* __restrict__ x, int
     hi += __shfl_down(hi, off, 64);
     lo += __shfl_down(lo, off, 64);
   }
+  // one atomic pair per workgroup (the 64-bit atomics on one address serialize)
+  __shared__ long long whi[4];
+  __shared__ unsigned long long wlo[4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd((unsigned long long*)hi_acc, (unsigned long long)hi);
-    atomicAdd(lo_acc, lo);
+    whi[w] = hi;
+    wlo[w] = lo;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)hi_acc, (unsigned long long)(whi[0] + whi[1] + whi[2] + whi[3]));
+    atomicAdd(lo_acc, wlo[0] + wlo[1] + wlo[2] + wlo[3]);
   }
 }
 
@@ -177,23 +190,23 @@ __global__ __launch_bounds__(256) void k_feats(const float* __restrict__ rank, i
 // Column sums of an nparts × m matrix, fixed order (deterministic).
 // Column sums of nparts rows of m doubles; blockIdx.y selects one of several such tables
 // (table y at part + y · nparts · m, its sums at out + y · m).
+// out[c] = Σ_r part[r · m + c]: one workgroup per column c (blockIdx.x) and job (blockIdx.y);
+// each thread sums rows t, t + 256, … in order, then a fixed tree (the same sums whatever m).
 __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int nparts, int m,
                                                    double* out) {
   part += (int64_t)blockIdx.y * nparts * m;
   out += (int64_t)blockIdx.y * m;
   __shared__ double sm[256];
-  for (int c = 0; c < m; ++c) {
-    double s = 0.0;
-    for (int r = threadIdx.x; r < nparts; r += 256) s += part[(int64_t)r * m + c];
-    sm[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-      if ((int)threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = sm[0];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < nparts; r += 256) s += part[(int64_t)r * m + c];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[c] = sm[0];
 }
 
 // f ← L⁻¹ (f − μ) in FP64, and the stack [1, f0..f3].
@@ -1880,11 +1893,11 @@ void launch_round_half(float* x, int64_t n, hipStream_t st) {
   k_round_half<<<min(blocks_for(n, 256), 8192u), 256, 0, st>>>(x, n);
 }
 void launch_maxabs(const float* x, int64_t n, uint32_t* out, hipStream_t st) {
-  k_maxabs<<<min(blocks_for(n, 256), 4096u), 256, 0, st>>>(x, n, out);
+  k_maxabs<<<min(blocks_for(n, 256), 512u), 256, 0, st>>>(x, n, out);
 }
 void launch_fixsum(const float* x, int64_t n, double scale, float mean, int squared, long long* hi,
                    unsigned long long* lo, hipStream_t st) {
-  k_fixsum<<<min(blocks_for(n, 256), 4096u), 256, 0, st>>>(x, n, scale, mean, squared, hi, lo);
+  k_fixsum<<<min(blocks_for(n, 256), 512u), 256, 0, st>>>(x, n, scale, mean, squared, hi, lo);
 }
 void launch_hist(const float* x, int64_t n, float mean, float spread, uint32_t* hist, hipStream_t st) {
   k_hist<<<min(blocks_for(n, 256), 2048u), 256, 0, st>>>(x, n, mean, spread, hist);
@@ -1900,7 +1913,7 @@ int launch_feats(const float* rank, int w, int h, int mode, float* feat, float* 
   return feats_blocks(w, h);
 }
 void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStream_t st) {
-  k_sum_parts<<<1, 256, 0, st>>>(part, nparts, m, out);
+  k_sum_parts<<<(unsigned)m, 256, 0, st>>>(part, nparts, m, out);
 }
 void launch_whiten(float* feat, float* stack, int64_t n, const WhitenArgs& a, hipStream_t st) {
   k_whiten<<<min(blocks_for(n, 256), 8192u), 256, 0, st>>>(feat, stack, n, a);
@@ -2224,13 +2237,13 @@ int jobs_blocks(int64_t max_pixels) {
 void launch_pose_moments_jobs(const PoseJob* jobs, int njobs, int blocks, double* part, double* sums,
                               hipStream_t st) {
   k_pose_moments_jobs<<<dim3(blocks, njobs), 256, 0, st>>>(jobs, part);
-  k_sum_parts<<<dim3(1, njobs), 256, 0, st>>>(part, blocks, 45, sums);
+  k_sum_parts<<<dim3(45, njobs), 256, 0, st>>>(part, blocks, 45, sums);
 }
 
 void launch_tangent_jobs(const TangentJob* jobs, int njobs, int blocks, double* part, double* sums,
                          hipStream_t st) {
   k_tangent_jobs<<<dim3(blocks, njobs), 256, 0, st>>>(jobs, part);
-  k_sum_parts<<<dim3(1, njobs), 256, 0, st>>>(part, blocks, 44, sums);
+  k_sum_parts<<<dim3(44, njobs), 256, 0, st>>>(part, blocks, 44, sums);
 }
 
 }  // namespace zc
