@@ -54,6 +54,45 @@ def _plan(three, rect):
     return {"n_cplx_a": (ra + 1) // 2, "n_cplx_b": (rb + 1) // 2, "n_out": (C + 1) // 2, "n_corr": C, "n_sat": T}
 
 
+def bench_refine(a, np, torch, zcmir_amd, synth):
+    """The refine loop (zc_climb: shift-map hops, Gauss–Newton blocks, roto-scale hops) on the
+    1024² map pair from a start pose 6 px and 2° off the truth: wall time per climb against the
+    GPU time of its launches (HIP events), i.e. what the host-driven loop costs over its kernels."""
+    import math
+    import time
+
+    mv, fx, Ht = synth.shift_pair((1024, 1024), (113, -57), seed=11, quantize=True)
+    t = math.radians(2.0)
+    C = np.array([[1, 0, 512.5], [0, 1, 512.5], [0, 0, 1.0]])
+    R = np.array([[math.cos(t), -math.sin(t), 0], [math.sin(t), math.cos(t), 0], [0, 0, 1]])
+    H0 = C @ R @ np.linalg.inv(C) @ Ht @ np.array([[1, 0, 6.0], [0, 1, -4.0], [0, 0, 1]])
+    reg = zcmir_amd.Registrar(half=False, symmetric=True, hop=True, hop_fm=True, g_iters=25)
+    reg.set_images(mv, fx, preprocess=False)
+    reg.pose = H0
+    c = reg.climb()   # warm-up: tilings, buffers
+    walls, gpu = [], []
+    for _ in range(a.refine_runs):
+        reg.pose = H0
+        torch.cuda.synchronize()
+        reg.profile(True)
+        t0 = time.perf_counter()
+        c = reg.climb()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        gpu.append(sum(tot for _, tot, _ in reg.profile_report().values()) * 1e-3)
+        reg.profile(False)
+    corners = np.array([[0.5, 0.5, 1], [1024.5, 0.5, 1], [1024.5, 1024.5, 1], [0.5, 1024.5, 1]]).T
+    p, q = c.H @ corners, Ht @ corners
+    err = float(np.max(np.hypot(*(p[:2] / p[2] - q[:2] / q[2]))))
+    reg.close()
+    w, g = float(np.median(walls)), float(np.median(gpu))
+    return {"metric": "refine (climb) wall time", "unit": "ms", "workload": "1024x1024 map pair, start 6 px and "
+            "2 deg off the truth, symmetric SMI, shift-map and roto-scale hops, Gauss-Newton blocks",
+            "ms_per_climb": round(1e3 * w, 2), "gpu_ms_per_climb": round(1e3 * g, 2),
+            "host_share": round(1 - g / w, 3), "iterations": int(c.iterations), "corner_err_px": round(err, 3),
+            "runs": a.refine_runs}
+
+
 def calibrate_copy(mb):
     """A plain device-to-device copy of `mb` MiB (the PMC calibration kernel)."""
     import torch
@@ -241,6 +280,7 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=10, help="timed sweeps on the 128² canvas (0: skip)")
     ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
     ap.add_argument("--prior-pairs", type=int, default=6, help="time-to-pose pairs, learned vs full (0: skip)")
+    ap.add_argument("--refine-runs", type=int, default=3, help="timed climbs on the 1024² pair (0: skip)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -353,6 +393,8 @@ def main():
                          **bench_sweep(a, np, torch, zcmir_amd, synth)}
     if a.prior_pairs > 0:
         line["prior"] = bench_prior(a, np, torch, zcmir_amd)
+    if a.refine_runs > 0:
+        line["refine"] = bench_refine(a, np, torch, zcmir_amd, synth)
     print(json.dumps(line))
 
 
