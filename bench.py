@@ -93,6 +93,34 @@ def bench_refine(a, np, torch, zcmir_amd, synth):
             "runs": a.refine_runs}
 
 
+def bench_rs(a, np, torch, zcmir_amd, synth):
+    """Roto-scale (log-polar) maps of a 1024² pair (moving = fixed rotated 37°, scaled 1.12),
+    symmetric (both directions), exact SMI, the default resolution cap: ms per map and its
+    kernels (HIP events)."""
+    import time
+
+    mv, fx, _ = synth.rotscale_pair((1024, 1024), theta_deg=37.0, scale=1.12, seed=2)
+    reg = zcmir_amd.Registrar(half=False, symmetric=True, exact=True)
+    reg.set_images(mv, fx, preprocess=False)
+    r = reg.rs_map(return_maps=False)   # warm-up: tilings, tables, buffers
+    torch.cuda.synchronize()
+    reg.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(a.rs_runs):
+        r = reg.rs_map(return_maps=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = reg.profile_report()
+    reg.profile(False)
+    reg.close()
+    top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:6]
+    return {"metric": "roto-scale map time", "unit": "ms", "workload": "1024x1024 pair, moving = fixed rotated "
+            "37 deg, scaled 1.12; symmetric (forward and inverse maps), exact SMI, default map_res",
+            "ms_per_map": round(1e3 * wall / a.rs_runs, 3), "n": r.n, "n_theta": r.n_th, "n_lam": r.n_lam,
+            "peak_dtheta_deg": round(float(np.degrees(r.dth)), 3), "peak_scale": round(float(np.exp(r.dsg)), 4),
+            "kernels_ms_per_map": {k: round(tot / a.rs_runs, 4) for k, (cnt, tot, _) in top}, "runs": a.rs_runs}
+
+
 def calibrate_copy(mb):
     """A plain device-to-device copy of `mb` MiB (the PMC calibration kernel)."""
     import torch
@@ -281,6 +309,7 @@ def main():
     ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
     ap.add_argument("--prior-pairs", type=int, default=6, help="time-to-pose pairs, learned vs full (0: skip)")
     ap.add_argument("--refine-runs", type=int, default=3, help="timed climbs on the 1024² pair (0: skip)")
+    ap.add_argument("--rs-runs", type=int, default=10, help="timed roto-scale maps on a 1024² pair (0: skip)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -395,6 +424,8 @@ def main():
         line["prior"] = bench_prior(a, np, torch, zcmir_amd)
     if a.refine_runs > 0:
         line["refine"] = bench_refine(a, np, torch, zcmir_amd, synth)
+    if a.rs_runs > 0:
+        line["rs_map"] = bench_rs(a, np, torch, zcmir_amd, synth)
     print(json.dumps(line))
 
 
