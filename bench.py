@@ -310,6 +310,7 @@ def main():
     ap.add_argument("--prior-pairs", type=int, default=6, help="time-to-pose pairs, learned vs full (0: skip)")
     ap.add_argument("--refine-runs", type=int, default=3, help="timed climbs on the 1024² pair (0: skip)")
     ap.add_argument("--rs-runs", type=int, default=10, help="timed roto-scale maps on a 1024² pair (0: skip)")
+    ap.add_argument("--sweep-batch", type=int, default=0, help="sweep candidates per batch (0: 64, the default)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -329,6 +330,8 @@ def main():
     three = a.corr == 28
     if a.fft_cols or a.fft_rows:
         zcmir_amd._load().zc_fft_tune(a.fft_rows, a.fft_cols)
+    if a.sweep_batch:
+        zcmir_amd._load().zc_sweep_tune(a.sweep_batch)
     size, shift = (1024, 1024), (113, -57)
     mv, fx, _ = synth.shift_pair(size, shift, seed=11, quantize=True)
     reg = zcmir_amd.Registrar(half=False, symmetric=False, smi_features="three" if three else "laplacian")

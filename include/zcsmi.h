@@ -192,7 +192,8 @@ int32_t zc_sweep_cells(void* e, const double H0[9], const uint32_t* cells, uint3
                        zc_sweep_peak* peaks, size_t cap, float* best, size_t nbest);
 /* the score map of the last sweep batch's first candidate; dims = (nx, ny) */
 int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
-/* candidates per sweep batch (0: 64); process-wide */
+/* candidates per sweep batch (0: 256; the reference's batches hold 64); process-wide; results
+ * do not depend on it */
 void zc_sweep_tune(uint32_t batch);
 
 /* stacks of uneven slice pairs (the role of python/zcmir/api.py:522-580 register_stack, batched):

@@ -79,7 +79,7 @@ zc::SweepOpts sweep_opts(Engine* e, zc::SweepGrid& gv) {
   o.mode = s.b("exact") ? 1 : 0;
   o.n_peak = 4;
   const int tb = zc::SweepTune::batch;
-  o.batch = tb > 0 ? (uint32_t)tb : 64;
+  o.batch = tb > 0 ? (uint32_t)tb : zc::kSweepBatch;
   o.min_overlap = clampf(s.f("min_overlap"), 0, 0.5);
   return o;
 }

@@ -157,7 +157,10 @@ struct FftTune {
   static inline std::atomic<int> rows{0}, cols{0};
 };
 
-// Sweep tuning: candidates per batch (0: 64, the reference's default).
+// Candidates per sweep batch by default: 256 (the reference uses 64; a batch only schedules the
+// candidates' independent maps, and 256 measured 47.0K against 44.7K poses/s on the 128² canvas).
+inline constexpr uint32_t kSweepBatch = 256;
+// Sweep tuning: candidates per batch (0: kSweepBatch).
 struct SweepTune {
   static inline std::atomic<int> batch{0};
 };
@@ -169,7 +172,7 @@ struct SweepOpts {
   bool full = false;     // one cell per fixed pixel (grid = ceil(D))
   uint32_t mode = 1;     // 0 global, 1 exact SMI
   uint32_t n_peak = 4;
-  uint32_t batch = 64;
+  uint32_t batch = kSweepBatch;
   double min_overlap = 0.05;
   // only these (rotation, scale) index pairs of the grid (shear and stretch index 0), in order
   const std::vector<std::array<uint32_t, 2>>* cells = nullptr;

@@ -55,7 +55,7 @@ std::vector<GnOut> Engine::stack_register(const std::vector<Mat3>& H0, uint32_t 
       o.grid = 64;
       o.mode = exact ? 1 : 0;
       o.n_peak = 4;
-      o.batch = 64;
+      o.batch = kSweepBatch;
       o.min_overlap = std::min(std::max(set.f("min_overlap"), 0.0), 0.5);
       const SweepOut r = sweep_sides(*stack[2 * i], *stack[2 * i + 1], o);
       const SweepPeak* best = nullptr;
