@@ -211,6 +211,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     put(col_lines((int)gr.ny));
   }
   put(plan);
+  put(FftRowTiling::v.load());   // the inverse row pass's kernel
   put(n_peak);
   put(ox);
   put(oy);
