@@ -465,7 +465,7 @@ void Engine::prepare(Side& sd, bool acf) {
 
 // Normalized feature autocorrelation (smi.wgsl:936-975, 1574-1580).
 void Engine::ensure_acf(Side& sd) {
-  const int nb = acf_blocks(sd.w, sd.h);
+  const int nb = acf_parts(sd.w, sd.h);
   parts.ensure((size_t)nb * 1156 * 8);
   sums.ensure(2048 * 8);
   LAUNCH("acf", launch_acf(sd.feat.as<float>(), sd.w, sd.h, parts.as<double>(), st));

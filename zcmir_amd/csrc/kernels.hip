@@ -228,8 +228,11 @@ __global__ __launch_bounds__(256) void k_whiten(float* feat, float* stack, int64
 }
 
 // Autocorrelation partial sums Σ_x f_i(x) f_i(x + τ), |τ| ≤ 8 (289 lags × 4 planes), over
-// 32×32 tiles with an 8-px halo (zero outside the image).
-constexpr int ACF_T = 32;
+// 32×32 tiles with an 8-px halo (zero outside the image, so whole tiles can be summed). Thread
+// (plane p, row lag dy, third q of the tile's rows) slides a 17-value window of row y + dy along
+// row y: each step reads one value of each row from LDS and updates all 17 column lags. One
+// partial-sum row per (tile, third): part[(tile · ACF_Q + q) · 1156 + lag · 4 + p].
+constexpr int ACF_T = 32, ACF_Q = 3;
 __global__ __launch_bounds__(256) void k_acf(const float* __restrict__ feat, int w, int h, double* part) {
   constexpr int S = ACF_T + 16;
   __shared__ float t[4][S][S + 1];
@@ -241,16 +244,36 @@ __global__ __launch_bounds__(256) void k_acf(const float* __restrict__ feat, int
     t[p][ly][lx] = (gx >= 0 && gy >= 0 && gx < w && gy < h) ? feat[p * n + (int64_t)gy * w + gx] : 0.0f;
   }
   __syncthreads();
-  const int blk = blockIdx.y * gridDim.x + blockIdx.x;
-  const int tw_ = min(ACF_T, w - x0), th_ = min(ACF_T, h - y0);
-  for (int c = threadIdx.x; c < 289 * 4; c += 256) {
-    const int lag = c >> 2, p = c & 3;
-    const int dx = lag % 17 - 8, dy = lag / 17 - 8;
-    float acc = 0.0f;
-    for (int yy = 0; yy < th_; ++yy)
-      for (int xx = 0; xx < tw_; ++xx) acc += t[p][yy + 8][xx + 8] * t[p][yy + 8 + dy][xx + 8 + dx];
-    part[(int64_t)blk * 1156 + c] = (double)acc;
+  const int id = threadIdx.x;
+  if (id >= 4 * 17 * ACF_Q) return;   // no barrier follows
+  const int q = id % ACF_Q, r = id / ACF_Q, dyi = r % 17, p = r / 17;
+  constexpr int ROWS = (ACF_T + ACF_Q - 1) / ACF_Q;
+  const int ya = q * ROWS, yb = min(ACF_T, ya + ROWS);
+  float acc[17];
+#pragma unroll
+  for (int i = 0; i < 17; ++i) acc[i] = 0.0f;
+  for (int yy = ya; yy < yb; ++yy) {
+    const float* r1 = &t[p][yy + 8][8];
+    const float* r2 = &t[p][yy + dyi][0];   // row yy + 8 + dy, columns from x − 8
+    float win[17];
+#pragma unroll
+    for (int i = 0; i < 17; ++i) win[i] = r2[i];
+#pragma unroll
+    for (int xx = 0; xx < ACF_T; ++xx) {
+      const float a = r1[xx];
+#pragma unroll
+      for (int i = 0; i < 17; ++i) acc[i] += a * win[i];
+      if (xx + 1 < ACF_T) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) win[i] = win[i + 1];
+        win[16] = r2[xx + 17];
+      }
+    }
   }
+  const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+  double* out = part + ((int64_t)blk * ACF_Q + q) * 1156;
+#pragma unroll
+  for (int i = 0; i < 17; ++i) out[(dyi * 17 + i) * 4 + p] = (double)acc[i];
 }
 
 // ── maps: canvas ──────────────────────────────────────────────────────────────────────────
@@ -1918,7 +1941,7 @@ void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStr
 void launch_whiten(float* feat, float* stack, int64_t n, const WhitenArgs& a, hipStream_t st) {
   k_whiten<<<min(blocks_for(n, 256), 8192u), 256, 0, st>>>(feat, stack, n, a);
 }
-int acf_blocks(int w, int h) { return ((w + ACF_T - 1) / ACF_T) * ((h + ACF_T - 1) / ACF_T); }
+int acf_parts(int w, int h) { return ((w + ACF_T - 1) / ACF_T) * ((h + ACF_T - 1) / ACF_T) * ACF_Q; }
 void launch_acf(const float* feat, int w, int h, double* part, hipStream_t st) {
   dim3 g((w + ACF_T - 1) / ACF_T, (h + ACF_T - 1) / ACF_T);
   k_acf<<<g, 256, 0, st>>>(feat, w, h, part);

@@ -38,7 +38,7 @@ int launch_feats(const float* rank, int w, int h, int mode, float* feat, float* 
 int feats_blocks(int w, int h);
 void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStream_t st);
 void launch_whiten(float* feat, float* stack, int64_t n, const WhitenArgs& a, hipStream_t st);
-int acf_blocks(int w, int h);
+int acf_parts(int w, int h);   // partial-sum rows of launch_acf (1156 doubles each)
 void launch_acf(const float* feat, int w, int h, double* part, hipStream_t st);
 
 // ── maps ──
