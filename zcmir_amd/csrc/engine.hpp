@@ -163,6 +163,7 @@ inline constexpr uint32_t kSweepBatch = 256;
 // Sweep tuning: candidates per batch (0: kSweepBatch).
 struct SweepTune {
   static inline std::atomic<int> batch{0};
+  static inline std::atomic<int> budget_gb{32};   // work buffers of one batch, GiB
 };
 
 struct SweepOpts {
