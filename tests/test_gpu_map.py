@@ -229,3 +229,43 @@ def test_row_tilings_of_the_inverse_pass(reg_factory):
     for rows in (1, 3):
         assert np.array_equal(maps[rows].view(np.uint32), maps[0].view(np.uint32)), rows
     assert np.abs(maps[2] - maps[0]).max() <= 1e-5 * np.abs(maps[0]).max()
+
+
+@pytest.mark.parametrize("mode", ["laplacian", "three"])
+def test_shift_map_2048_full_resolution_sampled_lags(reg_factory, mode):
+    """2048² pair (8-bit, shift (113, -57)), 45 and 28 correlations: the full-resolution map
+    (4096² FFT, rectangle sums over 2048-wide tables) at a dozen lags against the float64 oracle's
+    score of the sums taken directly at each lag (a whole float64 4096² map would take minutes);
+    the peak at the shift."""
+    mv, fx, _ = _pair((2048, 2048), (113, -57), seed=12, quantize=True)
+    reg = reg_factory(half=False, smi_features=mode)
+    reg.set_images(mv, fx, preprocess=False)
+    sm = reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096)
+    assert (sm.n, sm.ny, sm.cw, sm.ch) == (4096, 4096, 2048, 2048)
+    assert (sm.dx, sm.dy) == (113.0, -57.0)
+    pa, pb = O.prepare(mv, mode), O.prepare(fx, mode)
+    ox, oy, W, Hh = O.reg_frame(np.eye(3), 2048, 2048, 2048, 2048)
+    ra = O.real_planes(O.warp_stack(pa["stack"], O.canvas_h(np.eye(3), ox, oy), W, Hh), True)
+    rb = O.real_planes(O.paste_stack(pb["stack"], ox, oy, W, Hh), True)
+    pairs = O.corr_list(True)
+
+    def sums(tx, ty):   # Σ_x a(x + t) b(x) for all 45 pairs at lag t
+        ys, ye, xs, xe = max(0, -ty), min(Hh, Hh - ty), max(0, -tx), min(W, W - tx)
+        A = ra[:, ys + ty:ye + ty, xs + tx:xe + tx].reshape(len(ra), -1)
+        B = rb[:, ys:ye, xs:xe].reshape(len(rb), -1)
+        G = A @ B.T
+        return np.array([G[a, b] for a, b in pairs])
+
+    lags = [(0, 0), (113, -57), (112, -57), (113, -56), (30, 40), (-700, 900), (1500, -300),
+            (-1200, -1200), (1000, 1000), (-1850, 120), (400, -1700)]
+    corr = np.stack([sums(tx, ty) for tx, ty in lags], axis=1)
+    n0 = corr[0, 0]
+    om = O.combine(corr, exact=True, nf=3 if mode == "three" else 4, zero_n=n0)
+    floor = max(O.MIN_N, 0.08 * n0)
+    scale = np.abs(sm.map).max()
+    for k, (tx, ty) in enumerate(lags):
+        if abs(corr[0, k] - floor) <= 1e-3 * floor + 1.0:
+            continue   # at the overlap floor either side may round to 0
+        g = sm.map[ty % sm.ny, tx % sm.n]
+        assert abs(g - om[k]) <= MAP_TOL * scale, ((tx, ty), g, om[k], scale)
+    assert abs(sm.map[-57 % sm.ny, 113] - scale) == 0
