@@ -93,6 +93,32 @@ def bench_refine(a, np, torch, zcmir_amd, synth):
             "runs": a.refine_runs}
 
 
+def bench_map2048(a, np, torch, zcmir_amd, synth):
+    """The 2048² pair of the large-map workload (8-bit, shift (113, -57)) at full resolution
+    (4096² FFT, 45 correlations, rectangle-sum plan): maps/s and the per-kernel times (HIP events)."""
+    import time
+
+    mv, fx, _ = synth.shift_pair((2048, 2048), (113, -57), seed=12, quantize=True)
+    reg = zcmir_amd.Registrar(half=False, symmetric=False)
+    reg.set_images(mv, fx, preprocess=False)
+    sm = reg.shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, return_map=False)
+    if (sm.dx, sm.dy) != (113.0, -57.0) or (sm.n, sm.ny) != (4096, 4096):
+        raise SystemExit(f"2048 map workload check failed: peak ({sm.dx}, {sm.dy}), grid {sm.n}x{sm.ny}")
+    reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=2)
+    torch.cuda.synchronize()
+    reg.profile(True)
+    t0 = time.perf_counter()
+    reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=a.map2048_steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = reg.profile_report()
+    reg.profile(False)
+    reg.close()
+    return {"metric": "SMI maps/s @2048^2", "unit": "maps/s", "value": round(a.map2048_steps / wall, 2),
+            "ms_per_map": round(1e3 * wall / a.map2048_steps, 3), "fft": [4096, 4096],
+            "kernels_avg_ms": {k: round(avg, 4) for k, (cnt, tot, avg) in sorted(prof.items(), key=lambda kv: -kv[1][1])}}
+
+
 def bench_rs(a, np, torch, zcmir_amd, synth):
     """Roto-scale (log-polar) maps of a 1024² pair (moving = fixed rotated 37°, scaled 1.12),
     symmetric (both directions), exact SMI, the default resolution cap: ms per map and its
@@ -354,6 +380,7 @@ def main():
     ap.add_argument("--rs-runs", type=int, default=10, help="timed roto-scale maps on a 1024² pair (0: skip)")
     ap.add_argument("--sweep-batch", type=int, default=0, help="sweep candidates per batch (0: 256, the default)")
     ap.add_argument("--stack-slices", type=int, default=64, help="slices of the uneven stack workload (0: skip)")
+    ap.add_argument("--map2048-steps", type=int, default=20, help="timed maps of the 2048² pair (0: skip)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -474,6 +501,8 @@ def main():
         line["rs_map"] = bench_rs(a, np, torch, zcmir_amd, synth)
     if a.stack_slices > 0:
         line["stack"] = bench_stack(a, np, torch, zcmir_amd, synth)
+    if a.map2048_steps > 0:
+        line["map_2048"] = bench_map2048(a, np, torch, zcmir_amd, synth)
     print(json.dumps(line))
 
 

@@ -1126,8 +1126,8 @@ __global__ __launch_bounds__(64 * kCarryWaves) void k_sat_carry(double* band, in
     }
 }
 
-// Exclusive prefix of G values per thread over the 256 threads of the block, each value its own
-// scan (lds: 4 · G doubles).
+// Exclusive prefix of G values per thread over the threads of the block, each value its own
+// scan (lds: G doubles per wave).
 template <int G>
 __device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1158,10 +1158,10 @@ __device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
 // blockIdx.x band, blockIdx.y a group of G tables (one read of the canvas for all of them);
 // thread t owns columns [t·E, t·E + E) for the scans; each finished row goes out through LDS so
 // that lane i stores column x + i (whole cache lines per store instruction).
-template <int PID, int E, int G, int K0>
+template <int PID, int NT, int E, int G, int K0>
 __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, int W, int H,
                                               const double* __restrict__ band, double* sat, double* lds,
-                                              double (*stage)[256 * E]) {
+                                              double (*stage)[NT * E]) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr int n_tab = P.n_sat, k0 = K0;
   const int b = blockIdx.x, nb = gridDim.x;
@@ -1194,7 +1194,7 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
 #pragma unroll
     for (int g = 0; g < G; ++g)
       if (k0 + g < n_tab)
-        for (int x = threadIdx.x; x <= W; x += 256) sat[(k0 + g) * SP + x] = 0.0;
+        for (int x = threadIdx.x; x <= W; x += NT) sat[(k0 + g) * SP + x] = 0.0;
   const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
   // the canvas values of the thread's columns, one row ahead of the scans (E <= 4: the registers
   // of a second row fit)
@@ -1242,26 +1242,27 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
     for (int g = 0; g < G; ++g) {
       if (k0 + g >= n_tab) continue;
       double* row = sat + (k0 + g) * SP + (int64_t)(y + 1) * SW;
-      for (int x = threadIdx.x; x < W; x += 256) row[x + 1] = stage[g][x];
+      for (int x = threadIdx.x; x < W; x += NT) row[x + 1] = stage[g][x];
       if (threadIdx.x == 0) row[0] = 0.0;
     }
   }
 #undef ZC_SAT_LOAD_ROW
 }
 
-// blockIdx.y: the group of G tables (tables G·y .. G·y + G − 1 of the plan)
-template <int PID, int E, int G>
-__global__ __launch_bounds__(256) void k_sat_rows(const float* __restrict__ canvas, int W, int H,
-                                                  const double* __restrict__ band, double* sat) {
-  __shared__ double lds[4 * G];
-  __shared__ double stage[G][256 * E];
+// blockIdx.y: the group of G tables (tables G·y .. G·y + G − 1 of the plan); NT threads of E
+// columns each cover rows of up to NT·E cells.
+template <int PID, int NT, int E, int G>
+__global__ __launch_bounds__(NT) void k_sat_rows(const float* __restrict__ canvas, int W, int H,
+                                                 const double* __restrict__ band, double* sat) {
+  __shared__ double lds[NT / 64 * G];
+  __shared__ double stage[G][NT * E];
   static_assert(kPlans[PID].n_sat <= 5 * G, "at most five groups");
   switch (blockIdx.y) {
-    case 0: sat_rows_body<PID, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
-    case 1: sat_rows_body<PID, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
-    case 2: sat_rows_body<PID, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
-    case 3: sat_rows_body<PID, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
-    default: sat_rows_body<PID, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
+    case 0: sat_rows_body<PID, NT, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
+    case 1: sat_rows_body<PID, NT, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
+    case 2: sat_rows_body<PID, NT, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
+    case 3: sat_rows_body<PID, NT, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
+    default: sat_rows_body<PID, NT, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
   }
 }
 
@@ -1275,8 +1276,8 @@ static void sat_build_P(const float* canvas, int W, int H, double* band, double*
   k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 64 * kCarryWaves, 0, st>>>(band, W, nb);
   constexpr int G = 3;   // tables per workgroup
   const dim3 g(nb, (n_tab + G - 1) / G);
-  if (W <= 256 * 4) k_sat_rows<PID, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
-  else k_sat_rows<PID, 8, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
+  if (W <= 256 * 4) k_sat_rows<PID, 256, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
+  else k_sat_rows<PID, 512, 4, G><<<g, 512, 0, st>>>(canvas, W, H, band, sat);
 }
 
 bool launch_sat_build(int plan, const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
