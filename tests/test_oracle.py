@@ -272,3 +272,42 @@ def test_pose_moments_brute_force_translation():
             va, vb = fa[:, my, mx], fb[:, y, x]
             s += O.moments(va[:, None], vb[:, None], np.array([wt]))
     assert np.allclose(st, s, rtol=1e-12, atol=1e-9)
+
+
+# ── pack_resample (smi.wgsl:626-700): closed forms of the k×k bilinear box filter ───────────────
+def test_pack_resample_integer_factors_are_block_means():
+    """At an integer factor f (k = f taps per axis) every tap lands on a pixel centre
+    (x = f·gx + i exactly), so each cell is the plain f×f block mean."""
+    rng = np.random.default_rng(5)
+    for f in (2, 3, 4):
+        canvas = rng.uniform(-1, 1, (5, 6 * f, 7 * f))
+        got = O.pack_resample(canvas, 7, 6)
+        want = canvas.reshape(5, 6, f, 7, f).mean(axis=(2, 4))
+        assert np.abs(got - want).max() < 1e-12, f
+
+
+def test_pack_resample_reproduces_linear_ramps():
+    """Bilinear taps reproduce a linear function, and the taps of a cell are symmetric about its
+    centre, so a ramp a·x + b·y + c comes back at x = (gx + ½)·fx − ½ (and y alike) wherever no
+    tap is clamped at the border — for non-integer factors and for k clamped at 8."""
+    for (H, W, ch, cw) in ((90, 100, 37, 41), (64, 100, 5, 5), (33, 47, 33, 20)):
+        y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+        a, b, c = 0.37, -1.3, 2.0
+        canvas = np.stack([a * x + b * y + c] * 5)
+        got = O.pack_resample(canvas, cw, ch)
+        fx, fy = W / cw, H / ch
+        kx, ky = min(max(math.ceil(fx - 1e-4), 1), 8), min(max(math.ceil(fy - 1e-4), 1), 8)
+        gy, gx = np.mgrid[0:ch, 0:cw].astype(np.float64)
+        want = a * ((gx + 0.5) * fx - 0.5) + b * ((gy + 0.5) * fy - 0.5) + c
+        inside = (((gx + 0.5 / kx) * fx - 0.5 >= 0) & ((gx + 1 - 0.5 / kx) * fx - 0.5 <= W - 1)
+                  & ((gy + 0.5 / ky) * fy - 0.5 >= 0) & ((gy + 1 - 0.5 / ky) * fy - 0.5 <= H - 1))
+        assert inside.sum() > 0.5 * inside.size
+        assert np.abs(got[:, inside] - want[inside]).max() < 1e-9, (H, W, ch, cw)
+
+
+def test_pack_resample_constant_and_identity():
+    rng = np.random.default_rng(6)
+    canvas = np.full((5, 50, 70), 0.625)
+    assert np.abs(O.pack_resample(canvas, 23, 17) - 0.625).max() < 1e-12
+    canvas = rng.uniform(0, 1, (5, 20, 30))
+    assert np.array_equal(O.pack_resample(canvas, 30, 20), canvas)
