@@ -269,3 +269,17 @@ def test_shift_map_2048_full_resolution_sampled_lags(reg_factory, mode):
         g = sm.map[ty % sm.ny, tx % sm.n]
         assert abs(g - om[k]) <= MAP_TOL * scale, ((tx, ty), g, om[k], scale)
     assert abs(sm.map[-57 % sm.ny, 113] - scale) == 0
+
+
+@pytest.mark.parametrize("size,shift,exact,mode", [((97, 131), (5, -3), True, "laplacian"),
+                                                   ((131, 97), (-7, 4), False, "three"),
+                                                   ((33, 250), (11, 2), True, "three")])
+def test_shift_map_ragged_full_resolution(reg_factory, size, shift, exact, mode):
+    """Odd, ragged pairs at one cell per pixel: odd summed-area table widths, power-of-two FFTs
+    well above 2 W − 1, a thin 33-row image (bands shorter than the table)."""
+    mv, fx, _ = _pair(size, shift, seed=sum(size))
+    sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), exact=exact, mode=mode, cap=1024)
+    assert (sm.cw, sm.ch) == tuple(sm.canvas)
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same and (sm.dx, sm.dy) == (float(shift[0]), float(shift[1]))
