@@ -576,10 +576,10 @@ uint32_t zc_fit_check(void* e, const double*) {
   return 7;   // "it fails the sanity check": keypoint fits are not part of this library
 }
 void zc_tune_parts(uint32_t, uint32_t, uint32_t, uint32_t) {}
-// exports.zig:95: lines per workgroup of the batched FFT for rows / columns (0: default), for
-// FFTs run from now on (an unsupported value makes those FFTs fail with UnsupportedFftLength)
+// exports.zig:95: tilings of the batched FFTs for rows / columns (0: default), for FFTs run from
+// now on. A column value a length cannot take makes zc_fft2 fail with UnsupportedFftLength; maps
+// and sweeps then use the default tiling. A row value above 4 is the default.
 void zc_fft_tune(uint32_t rows, uint32_t cols) {
-  zc::FftTune::rows = (int)rows;
   zc::FftRowTiling::v = rows <= 4 ? (int)rows : 0;
   zc::FftTune::cols = (int)cols;
 }

@@ -152,9 +152,10 @@ struct Side {
   bool half = false;
 };
 
-// FFT tuning (zc_fft_tune): lines per workgroup for rows / columns, 0 = the default.
+// Column-FFT tiling forced by zc_fft_tune (0: the autotuned or built-in one); the row tiling is
+// FftRowTiling (kernels.hpp).
 struct FftTune {
-  static inline std::atomic<int> rows{0}, cols{0};
+  static inline std::atomic<int> cols{0};
 };
 
 // Candidates per sweep batch by default: 256 (the reference uses 64; a batch only schedules the
