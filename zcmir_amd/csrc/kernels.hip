@@ -1924,7 +1924,7 @@ void launch_fixsum(const float* x, int64_t n, double scale, float mean, int squa
   k_fixsum<<<min(blocks_for(n, 256), 512u), 256, 0, st>>>(x, n, scale, mean, squared, hi, lo);
 }
 void launch_hist(const float* x, int64_t n, float mean, float spread, uint32_t* hist, hipStream_t st) {
-  k_hist<<<min(blocks_for(n, 256), 2048u), 256, 0, st>>>(x, n, mean, spread, hist);
+  k_hist<<<min(blocks_for(n, 256), 256u), 256, 0, st>>>(x, n, mean, spread, hist);
 }
 void launch_rank(const float* x, int64_t n, float mean, float spread, const uint32_t* hist, const uint32_t* cdf,
                  float* rank, hipStream_t st) {
