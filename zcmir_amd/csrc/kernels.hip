@@ -187,11 +187,10 @@ __global__ __launch_bounds__(256) void k_feats(const float* __restrict__ rank, i
   block_sum<double, 14, 256>(acc, part + (int64_t)blk * 14, tid);
 }
 
-// Column sums of an nparts × m matrix, fixed order (deterministic).
-// Column sums of nparts rows of m doubles; blockIdx.y selects one of several such tables
-// (table y at part + y · nparts · m, its sums at out + y · m).
-// out[c] = Σ_r part[r · m + c]: one workgroup per column c (blockIdx.x) and job (blockIdx.y);
-// each thread sums rows t, t + 256, … in order, then a fixed tree (the same sums whatever m).
+// Column sums of nparts rows of m doubles, in a fixed order (deterministic); blockIdx.y selects
+// one of several such tables (table y at part + y · nparts · m, its sums at out + y · m).
+// out[c] = Σ_r part[r · m + c]: one workgroup per column c (blockIdx.x); each thread sums rows
+// t, t + 256, … in order, then a fixed tree.
 __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int nparts, int m,
                                                    double* out) {
   part += (int64_t)blockIdx.y * nparts * m;
