@@ -1,0 +1,530 @@
+// Map kernels, score side: per-lag whitened scores (FFT and rectangle-sum plans, roto-scale),
+// summed-area tables and the argmax.
+#include "kdev.hpp"
+
+namespace zc {
+
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr, float* map, int64_t nn,
+                                                 CombineArgs a, float* arg_part) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  auto C = [&](int num, int64_t t) -> double { return (double)corr[(int64_t)P.kept_of[num] * nn + t]; };
+  const double n0 = C(0, 0);
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
+    const float sv = (float)lag_score<PID>(C, t, floor_n, a, calib);
+    map[t] = sv;
+    if (better(sv, (uint32_t)t, bv, bi)) {
+      bv = sv;
+      bi = (uint32_t)t;
+    }
+  }
+  // block argmax
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+// ── rectangle sums ────────────────────────────────────────────────────────────────────────
+// When the fixed side's weight plane is the indicator of the rectangle R (the fixed image pasted
+// at full resolution), correlation (a, 0) at lag t is the sum of plane a over R + t clipped to the
+// canvas: four reads of plane a's summed-area table S(y, x) = sum over rows < y, columns < x
+// ((H+1) x (W+1) doubles per plane). The tables are built in three passes over bands of kSatBand
+// rows: every table's column sums per band (k_sat_band: one thread per column, all tables from
+// one read of the canvas), their exclusive prefix over bands (k_sat_carry), then every
+// (band, table) by a workgroup that scans its rows with a running column carry (k_sat_rows).
+constexpr int kSatBand = 8;   // rows per band
+
+// The real planes of a canvas stack sample in FP64: w, w f_i, and w f_i f_j = s_i s_j / s_0
+// (0 where s_0 <= 1e-6), the last with one reciprocal per sample.
+// pl is a compile-time constant at every call (the plan's table → plane map), so every index
+// below folds and the sample stays in registers.
+__device__ __forceinline__ double real_plane_d(int pl, const float* s, double inv0) {
+  if (pl < 5) return (double)s[pl];
+  if (s[0] <= 1e-6f) return 0.0;
+  constexpr int TI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+  constexpr int TJ[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+  const int k = pl - 5;
+  return (double)s[1 + TI[k]] * (double)s[1 + TJ[k]] * inv0;
+}
+
+__device__ __forceinline__ double canvas_inv0(const float* s) { return s[0] > 1e-6f ? 1.0 / (double)s[0] : 0.0; }
+
+__device__ __forceinline__ void canvas_values(const float* __restrict__ canvas, int64_t n, int64_t at, float* s) {
+#pragma unroll
+  for (int p = 0; p < 5; ++p) s[p] = canvas[p * n + at];
+}
+
+// blockIdx.x band, blockIdx.y a chunk of 256 columns: band[(k · nb + b) · W + x] = column sums of
+// table k's plane over the band's rows, for every table.
+template <int PID>
+__global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canvas, int W, int H, double* band) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int n_tab = P.n_sat;
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int x = blockIdx.y * 256 + threadIdx.x;
+  if (x >= W) return;
+  const int64_t n = (int64_t)W * H;
+  const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
+  double acc[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+  float sv[kSatBand][5];
+#pragma unroll
+  for (int i = 0; i < kSatBand; ++i) {   // every row's loads in flight together
+    const int y = min(y0 + i, H - 1);
+    canvas_values(canvas, n, (int64_t)y * W + x, sv[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < kSatBand; ++i) {
+    if (y0 + i < y1) {
+      const double inv0 = canvas_inv0(sv[i]);
+#pragma unroll
+      for (int k = 0; k < 15; ++k)
+        if (k < n_tab) acc[k] += real_plane_d(P.sat_plane[k], sv[i], inv0);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k)
+    if (k < n_tab) band[((int64_t)k * nb + b) * W + x] = acc[k];
+}
+
+// In place: band sums → exclusive prefix over bands, per (table, column). blockIdx.x: table ·
+// column chunks of 64; the block's 16 waves take a sixteenth of the bands each.
+constexpr int kCarryWaves = 16;
+__global__ __launch_bounds__(64 * kCarryWaves) void k_sat_carry(double* band, int W, int nb) {
+  __shared__ double seg[kCarryWaves][64];
+  const int chunks = (W + 63) / 64;
+  const int k = blockIdx.x / chunks, x = (blockIdx.x - k * chunks) * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
+  const int per = (nb + kCarryWaves - 1) / kCarryWaves, b0 = q * per, b1 = min(nb, b0 + per);
+  double* col = band + (int64_t)k * nb * W + x;
+  double tot = 0.0;
+  if (x < W)
+    for (int b = b0; b < b1; ++b) tot += col[(int64_t)b * W];
+  seg[q][threadIdx.x & 63] = tot;
+  __syncthreads();
+  double acc = 0.0;
+  for (int j = 0; j < q; ++j) acc += seg[j][threadIdx.x & 63];
+  if (x < W)
+    for (int b = b0; b < b1; ++b) {
+      const double v = col[(int64_t)b * W];
+      col[(int64_t)b * W] = acc;
+      acc += v;
+    }
+}
+
+// Exclusive prefix of G values per thread over the threads of the block, each value its own
+// scan (lds: G doubles per wave).
+template <int G>
+__device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double inc[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) inc[g] = v[g];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const double u = __shfl_up(inc[g], o, 64);
+      if (lane >= o) inc[g] += u;
+    }
+  }
+  if (lane == 63)
+#pragma unroll
+    for (int g = 0; g < G; ++g) lds[w * G + g] = inc[g];
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    double off = 0.0;
+    for (int j = 0; j < w; ++j) off += lds[j * G + g];
+    v[g] = off + inc[g] - v[g];
+  }
+  __syncthreads();   // lds is reused by the next scan
+}
+
+// blockIdx.x band, blockIdx.y a group of G tables (one read of the canvas for all of them);
+// thread t owns columns [t·E, t·E + E) for the scans; each finished row goes out through LDS so
+// that lane i stores column x + i (whole cache lines per store instruction).
+template <int PID, int NT, int E, int G, int K0>
+__device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, int W, int H,
+                                              const double* __restrict__ band, double* sat, double* lds,
+                                              double (*stage)[NT * E]) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int n_tab = P.n_sat, k0 = K0;
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int xs = threadIdx.x * E;
+  const int64_t n = (int64_t)W * H, SW = W + 1, SP = SW * (H + 1);
+  constexpr int pl[G] = {P.sat_plane[k0 < n_tab ? k0 : 0], P.sat_plane[k0 + 1 < n_tab ? k0 + 1 : 0],
+                         P.sat_plane[k0 + 2 < n_tab ? k0 + 2 : 0]};
+  static_assert(G == 3, "three tables per workgroup");
+  // column carry cc[g][e] = S_g(y, xs + e + 1): the row prefix of the earlier bands' column sums
+  double cc[G][E];
+  {
+    double tot[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      tot[g] = 0.0;
+      const double* crow = band + ((int64_t)min(k0 + g, n_tab - 1) * nb + b) * W;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        tot[g] += xs + e < W ? crow[xs + e] : 0.0;
+        cc[g][e] = tot[g];
+      }
+    }
+    block_excl_scan<G>(tot, lds);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int e = 0; e < E; ++e) cc[g][e] += tot[g];
+  }
+  if (b == 0)
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < n_tab)
+        for (int x = threadIdx.x; x <= W; x += NT) sat[(k0 + g) * SP + x] = 0.0;
+  const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
+  // the canvas values of the thread's columns, one row ahead of the scans (E <= 4: the registers
+  // of a second row fit)
+  constexpr bool AHEAD = E <= 4;
+  float nxt[E][5];
+#define ZC_SAT_LOAD_ROW(YY)                                                                         \
+  _Pragma("unroll") for (int e = 0; e < E; ++e) {                                                    \
+    _Pragma("unroll") for (int p = 0; p < 5; ++p) nxt[e][p] = 0.0f;                                  \
+    if ((YY) < y1 && xs + e < W)                                                                     \
+      _Pragma("unroll") for (int p = 0; p < 5; ++p) nxt[e][p] = canvas[p * n + (int64_t)(YY) * W + xs + e]; \
+  }
+  if (AHEAD) ZC_SAT_LOAD_ROW(y0)
+  for (int y = y0; y < y1; ++y) {
+    if (!AHEAD) ZC_SAT_LOAD_ROW(y)
+    float cur[E][5];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+#pragma unroll
+      for (int p = 0; p < 5; ++p) cur[e][p] = nxt[e][p];
+    if (AHEAD) ZC_SAT_LOAD_ROW(y + 1)
+    double v[G][E];
+    double tot[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) tot[g] = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float* sv = cur[e];
+      const double inv0 = canvas_inv0(sv);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        tot[g] += real_plane_d(pl[g], sv, inv0);
+        v[g][e] = tot[g];
+      }
+    }
+    block_excl_scan<G>(tot, lds);   // its barriers also order the previous row's stage reads
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        cc[g][e] += tot[g] + v[g][e];
+        stage[g][xs + e] = cc[g][e];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (k0 + g >= n_tab) continue;
+      double* row = sat + (k0 + g) * SP + (int64_t)(y + 1) * SW;
+      for (int x = threadIdx.x; x < W; x += NT) row[x + 1] = stage[g][x];
+      if (threadIdx.x == 0) row[0] = 0.0;
+    }
+  }
+#undef ZC_SAT_LOAD_ROW
+}
+
+// blockIdx.y: the group of G tables (tables G·y .. G·y + G − 1 of the plan); NT threads of E
+// columns each cover rows of up to NT·E cells.
+template <int PID, int NT, int E, int G>
+__global__ __launch_bounds__(NT) void k_sat_rows(const float* __restrict__ canvas, int W, int H,
+                                                 const double* __restrict__ band, double* sat) {
+  __shared__ double lds[NT / 64 * G];
+  __shared__ double stage[G][NT * E];
+  static_assert(kPlans[PID].n_sat <= 5 * G, "at most five groups");
+  switch (blockIdx.y) {
+    case 0: sat_rows_body<PID, NT, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
+    case 1: sat_rows_body<PID, NT, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
+    case 2: sat_rows_body<PID, NT, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
+    case 3: sat_rows_body<PID, NT, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
+    default: sat_rows_body<PID, NT, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
+  }
+}
+
+int sat_bands(int H) { return (H + kSatBand - 1) / kSatBand; }
+
+template <int PID>
+static void sat_build_P(const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+  constexpr int n_tab = kPlans[PID].n_sat;
+  const int nb = sat_bands(H);
+  k_sat_band<PID><<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, band);
+  k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 64 * kCarryWaves, 0, st>>>(band, W, nb);
+  constexpr int G = 3;   // tables per workgroup
+  const dim3 g(nb, (n_tab + G - 1) / G);
+  if (W <= 256 * 4) k_sat_rows<PID, 256, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
+  else k_sat_rows<PID, 512, 4, G><<<g, 512, 0, st>>>(canvas, W, H, band, sat);
+}
+
+bool launch_sat_build(int plan, const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+  if (W < 1 || H < 1 || W > 256 * 8) return false;
+  switch (plan) {
+    case 4: sat_build_P<4>(canvas, W, H, band, sat, st); return true;
+    case 5: sat_build_P<5>(canvas, W, H, band, sat, st); return true;
+    case 6: sat_build_P<6>(canvas, W, H, band, sat, st); return true;
+    case 7: sat_build_P<7>(canvas, W, H, band, sat, st); return true;
+    default: return false;
+  }
+}
+
+// Table offsets of the rectangle R + (dx, dy) clipped to the canvas for lag index t.
+struct SatIdx {
+  int64_t o00, o01, o10, o11;   // S(y0, x0), S(y0, x1), S(y1, x0), S(y1, x1)
+  bool empty;
+  bool r0, c0;                  // y0 = 0 / x0 = 0: those corners lie on the table's zero row / column
+};
+
+__device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
+  const int px = (int)(t & (v.nx - 1)), py = (int)(t >> v.lx);
+  const int dx = px <= v.nx / 2 ? px : px - v.nx;
+  const int dy = py <= v.ny / 2 ? py : py - v.ny;
+  const int x0 = min(max(v.bx0 + dx, 0), v.W), x1 = min(max(v.bx1 + dx, 0), v.W);
+  const int y0 = min(max(v.by0 + dy, 0), v.H), y1 = min(max(v.by1 + dy, 0), v.H);
+  SatIdx q;
+  const int64_t SW = v.W + 1;
+  q.empty = x1 <= x0 || y1 <= y0;
+  q.r0 = y0 == 0;
+  q.c0 = x0 == 0;
+  q.o00 = y0 * SW + x0;
+  q.o01 = y0 * SW + x1;
+  q.o10 = y1 * SW + x0;
+  q.o11 = y1 * SW + x1;
+  return q;
+}
+
+// Corners on the zero row or column are not read (half the lags of a map have y0 = 0 or x0 = 0).
+__device__ __forceinline__ double sat_sum(const double* __restrict__ S, const SatIdx& q) {
+  if (q.empty) return 0.0;
+  const double s11 = S[q.o11];
+  const double s01 = q.r0 ? 0.0 : S[q.o01];
+  const double s10 = q.c0 ? 0.0 : S[q.o10];
+  const double s00 = (q.r0 || q.c0) ? 0.0 : S[q.o00];
+  return (s11 - s01) - (s10 - s00);
+}
+
+// k_combine for the rectangle plans: correlations with sat_of >= 0 come from the tables.
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ corr, float* map, int64_t nn,
+                                                      CombineArgs a, SatView sv, float* arg_part) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  const int64_t SP = (int64_t)(sv.W + 1) * (sv.H + 1);
+  SatIdx q = sat_idx(sv, 0);
+  auto C = [&](int num, int64_t t) -> double {
+    if (P.sat_of[num] >= 0) return sat_sum(sv.sat + P.sat_of[num] * SP, q);
+    return (double)corr[(int64_t)P.kept_of[num] * nn + t];
+  };
+  const double n0 = C(0, 0);
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
+    q = sat_idx(sv, t);
+    const float s = (float)lag_score<PID & 3>(C, t, floor_n, a, calib);
+    map[t] = s;
+    if (better(s, (uint32_t)t, bv, bi)) {
+      bv = s;
+      bi = (uint32_t)t;
+    }
+  }
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+// ── roto-scale maps: per-lag score and finish ─────────────────────────────────────────────
+// Lag t of the n x n map: row k (signed, lambda), column kx (theta). The correlations were taken
+// on a px x py grid: px = n (circular in theta) or a padded length >= 2n, whose two linear lags kx
+// and kx - n fold into the circular one; py >= n (linear in lambda). Each correlation is scaled by
+// e^(k dlam) dlam dth (extract_pair's area element, smi.wgsl:778-796).
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine_rs(const float* __restrict__ corr, float* map, RsArgs r,
+                                                    CombineArgs a) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  const int64_t nn = (int64_t)r.px * r.py, total = (int64_t)r.n * r.n;
+  int64_t base = 0, base2 = -1;
+  double sc = 0.0;
+  auto at = [&](int64_t t) {
+    const int ky = (int)(t / r.n), kx = (int)(t - (int64_t)ky * r.n);
+    const int k = ky > r.n / 2 ? ky - r.n : ky;
+    const int row = k < 0 ? k + r.py : k;
+    base = (int64_t)row * r.px + kx;
+    base2 = (r.px != r.n && kx > 0) ? base - r.n + r.px : -1;
+    sc = exp((double)k * r.dlam) * r.dlam * r.dth;
+  };
+  auto C = [&](int num, int64_t) -> double {
+    const float* c = corr + (int64_t)P.kept_of[num] * nn;
+    double v = (double)c[base];
+    if (base2 >= 0) v += (double)c[base2];
+    return v * sc;
+  };
+  at(0);
+  const double n0 = C(0, 0);
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    at(t);
+    map[t] = (float)lag_score<PID>(C, t, floor_n, a, calib);
+  }
+}
+
+// fwd averaged with inv at the mirrored lag (-y, -x), or (-y, x) when !mirror; rows whose applied
+// scale exp(-k dlam) lies outside [e^lo, e^hi] zeroed (smi.wgsl:1708-1731); block argmax partials.
+__global__ __launch_bounds__(256) void k_rs_finish(const float* __restrict__ fwd, const float* __restrict__ inv,
+                                                   int n, int mirror, double dlam, double lo, double hi, float* out,
+                                                   float* arg_part) {
+  const int64_t total = (int64_t)n * n;
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int y = (int)(t / n), x = (int)(t - (int64_t)y * n);
+    float v = fwd[t];
+    if (inv) {
+      const int ym = (n - y) % n, xm = mirror ? (n - x) % n : x;
+      v = 0.5f * (v + inv[(int64_t)ym * n + xm]);
+    }
+    const int k = y > n / 2 ? y - n : y;
+    const double sg = -(double)k * dlam;
+    if (sg < lo || sg > hi) v = 0.0f;
+    out[t] = v;
+    if (better(v, (uint32_t)t, bv, bi)) {
+      bv = v;
+      bi = (uint32_t)t;
+    }
+  }
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_argmax_finish(const float* __restrict__ part, int nparts,
+                                                       const float* __restrict__ map, float* res) {
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int g = threadIdx.x; g < nparts; g += 256) {
+    const float v = part[2 * g];
+    const uint32_t i = __float_as_uint(part[2 * g + 1]);
+    if (i != 0xffffffffu && better(v, i, bv, bi)) {
+      bv = v;
+      bi = i;
+    }
+  }
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[0] = sv_[0];
+    res[1] = __uint_as_float(si_[0]);
+    res[2] = map[0];
+  }
+}
+
+
+
+int combine_blocks(int64_t nn) { return (int)std::min<int64_t>((nn + 255) / 256, 4096); }
+
+void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
+                    hipStream_t st, const SatView* sv) {
+  const int g = combine_blocks(nn);
+  if (plan >= 4 && !sv) throw std::runtime_error("rectangle plan without tables");
+  switch (plan) {
+    case 0: k_combine<0><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 1: k_combine<1><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 2: k_combine<2><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 3: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
+    case 4: k_combine_rect<4><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    case 5: k_combine_rect<5><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    case 6: k_combine_rect<6><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    default: k_combine_rect<7><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+  }
+}
+
+void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st) {
+  k_argmax_finish<<<1, 256, 0, st>>>(arg_part, nparts, map, res);
+}
+
+
+
+void launch_combine_rs(int plan, const float* corr, float* map, const RsArgs& r, const CombineArgs& a,
+                       hipStream_t st) {
+  const int g = combine_blocks((int64_t)r.n * r.n);
+  switch (plan) {
+    case 0: k_combine_rs<0><<<g, 256, 0, st>>>(corr, map, r, a); break;
+    case 1: k_combine_rs<1><<<g, 256, 0, st>>>(corr, map, r, a); break;
+    case 2: k_combine_rs<2><<<g, 256, 0, st>>>(corr, map, r, a); break;
+    default: k_combine_rs<3><<<g, 256, 0, st>>>(corr, map, r, a); break;
+  }
+}
+
+void launch_rs_finish(const float* fwd, const float* inv, int n, bool mirror, double dlam, double lo, double hi,
+                      float* out, float* arg_part, hipStream_t st) {
+  k_rs_finish<<<combine_blocks((int64_t)n * n), 256, 0, st>>>(fwd, inv, n, mirror ? 1 : 0, dlam, lo, hi, out, arg_part);
+}
+
+}  // namespace zc

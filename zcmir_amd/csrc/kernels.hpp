@@ -1,4 +1,4 @@
-// Host-side launchers of the SMI kernels (kernels.hip). All launches go on `st`; device
+// Host-side launchers of the SMI kernels (k_prep, k_pack, k_fft, k_ifft, k_score, k_pose, k_sweep .hip). All launches go on `st`; device
 // pointers only. Sizes are validated by the engine before any launch.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -36,7 +36,8 @@ void launch_rank(const float* x, int64_t n, float mean, float spread, const uint
 int launch_feats(const float* rank, int w, int h, int mode, float* feat, float* mag, double* part,
                  hipStream_t st);
 int feats_blocks(int w, int h);
-void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStream_t st);
+// batch: blockIdx.y selects consecutive groups of nparts x m partials and m outputs
+void launch_sum_parts(const double* part, int nparts, int m, double* out, hipStream_t st, int batch = 1);
 void launch_whiten(float* feat, float* stack, int64_t n, const WhitenArgs& a, hipStream_t st);
 int acf_parts(int w, int h);   // partial-sum rows of launch_acf (1156 doubles each)
 void launch_acf(const float* feat, int w, int h, double* part, hipStream_t st);
