@@ -27,6 +27,7 @@ def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
     traffic of its own role)."""
     N = nx * ny
     PA, PB, Q, C, T = plan["n_cplx_a"], plan["n_cplx_b"], plan["n_out"], plan["n_corr"], plan["n_sat"]
+    TB = plan.get("n_bsat", 0)
     out = {
         "warp_stack": 20 * S + 20 * canvas_px,
         "paste_stack": 20 * S + 20 * canvas_px,
@@ -35,23 +36,39 @@ def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
         "cmul": (PA + PB) * N * 8 + Q * N * 8,
         "colfft_inv": 2 * Q * N * 8,
         "rowifft_extract": Q * N * 8 + C * N * 4,
-        "combine": C * N * 4 + N * 4 + T * canvas_px * 8,
+        "combine": C * N * 4 + N * 4 + (T + TB) * canvas_px * 8,
     }
     if T:
-        # band sums and rows each read the canvas stack; the tables are written once
-        out["sat_build"] = 2 * 20 * canvas_px + T * canvas_px * 8
+        # band sums and rows each read the canvas stack; the tables are written once (per side)
+        out["sat_build"] = 2 * 20 * canvas_px * (2 if TB else 1) + (T + TB) * canvas_px * 8
     return out
 
 
-def _plan(three, rect):
+def survey_model_bytes(S, N, L):
+    """The minimal-pass traffic of an exact 45-correlation map with every correlation through the
+    FFT (8 complex spectra per side, 23 inverse outputs; two passes per 2-D FFT, the pack fused
+    into the forward row pass, the cross spectrum into the first inverse pass and the score into
+    the last): 40 S + 64·8 N + 16·23 N + 4 L bytes — 3.75 GB for a 1024² pair on a 2048² grid. A
+    map's rate against this fixed figure is comparable across plans."""
+    return 40 * S + 64 * 8 * N + 16 * 23 * N + 4 * L
+
+
+def _plan(three, rect, trans=False):
     """Spectra, inverse FFTs and correlations of the exact map plans (rectangle plans: the
-    correlations with the fixed weight plane come from summed-area tables)."""
+    correlations with the fixed weight plane come from summed-area tables; translation plans:
+    those with the moving weight plane too, from the fixed side's tables)."""
+    if trans:
+        nf = 3 if three else 4
+        C = nf * nf
+        return {"n_cplx_a": (nf + 1) // 2, "n_cplx_b": (nf + 1) // 2, "n_out": (C + 1) // 2, "n_corr": C,
+                "n_sat": 10 if three else 15, "n_bsat": 9 if three else 14, "n_real_a": nf, "n_real_b": nf}
     if not rect:
         R = 10 if three else 15
         C = 28 if three else 45
         return {"n_cplx_a": (R + 1) // 2, "n_cplx_b": (R + 1) // 2, "n_out": (C + 1) // 2, "n_corr": C, "n_sat": 0}
     ra, rb, C, T = (4, 9, 18, 10) if three else (5, 14, 30, 15)
-    return {"n_cplx_a": (ra + 1) // 2, "n_cplx_b": (rb + 1) // 2, "n_out": (C + 1) // 2, "n_corr": C, "n_sat": T}
+    return {"n_cplx_a": (ra + 1) // 2, "n_cplx_b": (rb + 1) // 2, "n_out": (C + 1) // 2, "n_corr": C, "n_sat": T,
+            "n_real_a": ra, "n_real_b": rb}
 
 
 def bench_refine(a, np, torch, zcmir_amd, synth):
@@ -426,7 +443,9 @@ def main():
     wall = t1 - t0
     maps_s = a.steps / wall
 
-    plan = _plan(three, (sm.cw, sm.ch) == tuple(sm.canvas))
+    rect = (sm.cw, sm.ch) == tuple(sm.canvas)
+    # the workload's pose is the identity: an integer translation onto the canvas
+    plan = _plan(three, rect, trans=rect)
     kb = kernel_bytes(plan, size[0] * size[1], sm.canvas[0] * sm.canvas[1], sm.n, sm.ny, sm.ch)
     kernels = {}
     for name, (cnt, tot, avg) in prof.items():
@@ -450,7 +469,9 @@ def main():
     # whole-pipeline minimum: features in, spectra written and read back once, inverse spectra,
     # the map, and the tables written and read once
     model_bytes = 40 * size[0] * size[1] + 32 * (plan["n_cplx_a"] + plan["n_cplx_b"]) * sm.n * sm.ny \
-        + 16 * plan["n_out"] * sm.n * sm.ny + 4 * sm.n * sm.ny + 16 * plan["n_sat"] * sm.canvas[0] * sm.canvas[1]
+        + 16 * plan["n_out"] * sm.n * sm.ny + 4 * sm.n * sm.ny \
+        + 16 * (plan["n_sat"] + plan.get("n_bsat", 0)) * sm.canvas[0] * sm.canvas[1]
+    survey_bytes = survey_model_bytes(size[0] * size[1], sm.n * sm.ny, sm.n * sm.ny)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -483,6 +504,8 @@ def main():
         "alg_bytes_per_map": total_alg,
         "model_bytes_per_map": model_bytes,
         "model_frac": round(model_bytes / (wall / a.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "survey_model_bytes_per_map": survey_bytes,
+        "survey_model_frac": round(survey_bytes / (wall / a.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_map"])},
         "device": reg.adapter,

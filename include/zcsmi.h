@@ -79,9 +79,10 @@ size_t zc_settings(void* e, char* buf, size_t cap);
 /* ── images and features ── */
 /* exports.zig:50: a work image, no preprocessing */
 int32_t zc_set_image(void* e, uint32_t side, const float* data, uint32_t w, uint32_t h);
-/* exports.zig:57; preprocess=1 is accepted only with clahe, band and invert off */
+/* exports.zig:57; preprocess=1: the settings' CLAHE, band pass and invert (gls.zig:462-559), then
+ * the f16 work image when half is on; preprocess=0: the image is the work image unchanged */
 int32_t zc_set_image_gray(void* e, uint32_t side, const float* data, uint32_t w, uint32_t h, uint32_t preprocess);
-/* exports.zig:63 (always preprocessed: same restriction) */
+/* exports.zig:63: RGBA8 → gray, always preprocessed as zc_set_image_gray with preprocess=1 */
 int32_t zc_set_image_rgba(void* e, uint32_t side, const uint8_t* rgba, uint32_t w, uint32_t h);
 /* exports.zig:238 */
 void zc_image_size(void* e, uint32_t side, uint32_t out[2]);
@@ -133,7 +134,9 @@ int32_t zc_climb(void* e, zc_climb_result* out);
 void zc_cancel(void* e);
 
 /* ── events, profiling, counters ── */
-/* exports.zig:412: JSON-lines events (none are queued by this build) */
+/* exports.zig:412 (events.zig:13-86): queued progress events as JSON lines, kinds "log",
+ * "trail", "pose" (zc_search, zc_climb) and "sweep" (zc_search's landscape); whole lines up to
+ * cap bytes, removed from the queue; 0 when none are left */
 size_t zc_events(void* e, char* buf, size_t cap);
 /* exports.zig:210: per-kernel timing on (1) / off (0); resets the report */
 int32_t zc_profile(void* e, uint32_t on);
@@ -195,6 +198,11 @@ int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
 /* candidates per sweep batch (0: 256; the reference's batches hold 64); process-wide; results
  * do not depend on it */
 void zc_sweep_tune(uint32_t batch);
+/* this library's: full-resolution maps at integer-translation poses from now on, 1 (default)
+ * through the translation plans (the moving weight plane is a rectangle too: the correlations
+ * with it come from summed-area tables, 16 of 45 through the FFT), 0 through the rectangle plans;
+ * process-wide; the two agree within FP32 rounding */
+void zc_map_path(uint32_t translation_plans);
 
 /* stacks of uneven slice pairs (the role of python/zcmir/api.py:522-580 register_stack, batched):
  * slice i = moving[i] (moving_wh[2i] x moving_wh[2i+1] f32 gray, taken as the work image) onto

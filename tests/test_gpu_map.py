@@ -283,3 +283,51 @@ def test_shift_map_ragged_full_resolution(reg_factory, size, shift, exact, mode)
     err, same, d = compare(sm.map, sm, om, info)
     assert err <= MAP_TOL, (err, d)
     assert same and (sm.dx, sm.dy) == (float(shift[0]), float(shift[1]))
+
+
+def test_shift_map_on_the_reference_grid(reg_factory):
+    """A full-resolution map whose reference plan takes 2^a·3^b lengths (576 x 648 for a 257 x 300
+    pair): the kernels run on power-of-two lengths, and the map, its size and its peak index come
+    back on the reference grid, in a buffer sized as the reference binding sizes it
+    (max(resolution, map_res, 1024)² floats), matching the oracle on that grid."""
+    mv, fx, _ = _pair((300, 257), (9, -4), seed=5)
+    sm, om, info = _map_case(reg_factory, mv, fx, np.eye(3), cap=1500)
+    assert (sm.n, sm.ny) == (576, 648) == (info["n"], info["ny"])   # 257 wide, 300 high
+    assert sm.map.shape == (648, 576)
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same and (sm.dx, sm.dy) == (9.0, -4.0)
+    assert sm.map.reshape(-1).argmax() == ((-4) % 648) * 576 + 9
+
+
+@pytest.mark.parametrize("size,H,exact,mode", [((180, 240), (9.0, -6.0), True, "laplacian"),
+                                               ((131, 97), (-4.0, 7.0), True, "three"),
+                                               ((150, 150), (0.0, 0.0), False, "laplacian"),
+                                               ((1024, 1024), (0.0, 0.0), True, "laplacian")])
+def test_translation_plans_match_oracle_and_rectangle_plans(reg_factory, size, H, exact, mode):
+    """Full-resolution maps at integer-translation poses take the translation plans (the moving
+    weight plane is a rectangle as well, so the correlations with it are rectangle sums of the
+    fixed side's tables and only the feature cross terms go through the FFT): the map matches the
+    float64 oracle and the rectangle plans' map (zc_map_path(0)) with the same peak, for shifted
+    canvases (negative offsets, the moving rectangle clipped), three features and global SMI."""
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    mv, fx, _ = _pair(size, (5, -3), seed=sum(size), quantize=size[0] >= 1024)
+    Hp = np.array([[1, 0, H[0]], [0, 1, H[1]], [0, 0, 1.0]])
+    cap = 2048 if size[0] >= 1024 else 1024
+    sm, om, info = _map_case(reg_factory, mv, fx, Hp, exact=exact, mode=mode, cap=cap)
+    assert (sm.cw, sm.ch) == tuple(sm.canvas)
+    err, same, d = compare(sm.map, sm, om, info)
+    assert err <= MAP_TOL, (err, d)
+    assert same
+    try:
+        lib.zc_map_path(0)
+        reg = reg_factory(half=False, smi_features=mode)
+        reg.set_images(mv, fx, preprocess=False)
+        rect = reg.shift_map(Hp, exact=exact, calibrated=False, resolution=cap)
+        reg.close()
+    finally:
+        lib.zc_map_path(1)
+    assert (rect.dx, rect.dy) == (sm.dx, sm.dy)
+    assert np.abs(rect.map - sm.map).max() <= MAP_TOL * np.abs(rect.map).max()

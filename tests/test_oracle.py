@@ -311,3 +311,79 @@ def test_pack_resample_constant_and_identity():
     assert np.abs(O.pack_resample(canvas, 23, 17) - 0.625).max() < 1e-12
     canvas = rng.uniform(0, 1, (5, 20, 30))
     assert np.array_equal(O.pack_resample(canvas, 30, 20), canvas)
+
+
+# ── the calibrated map's autocorrelation and correlation area (smi.wgsl:936-975, 1574-1595) ──
+def _acf_index(dx, dy):
+    return (dy + 8) * 17 + (dx + 8)
+
+
+def test_feature_acf_of_two_impulses_is_exact():
+    """A plane with two unit impulses d = (3, 2) apart has Σ a(x) a(x + t) = 2 at t = 0, 1 at
+    t = ±d and 0 at every other lag; divided by the overlap count (w − |dx|)(h − |dy|) and then
+    by the zero lag, lag ±d is w·h / (2 (w − 3)(h − 2)). A transposed lag index, a count of the
+    whole image or a lag read the wrong way round all fail this."""
+    h, w = 23, 31
+    feat = np.zeros((4, h, w))
+    for p in range(4):
+        y0, x0 = 4 + p, 6 + 2 * p
+        feat[p, y0, x0] = 1.0
+        feat[p, y0 + 2, x0 + 3] = 1.0
+    acf = O.feature_acf(feat)
+    want = np.zeros(289)
+    want[_acf_index(0, 0)] = 1.0
+    want[_acf_index(3, 2)] = want[_acf_index(-3, -2)] = w * h / (2.0 * (w - 3) * (h - 2))
+    for p in range(4):
+        assert np.abs(acf[:, p] - want).max() < 1e-12, p
+
+
+def test_feature_acf_of_constants_is_one_everywhere():
+    """A constant plane's lag products average to the constant squared over any overlap, so
+    every normalized lag is 1 (a sum divided by the whole image instead of the overlap would
+    fall to (w − 8)(h − 8)/(w h) at the corner lags)."""
+    feat = np.stack([np.full((12, 14), v) for v in (1.0, -2.5, 0.25, 3.0)])
+    assert np.abs(O.feature_acf(feat) - 1.0).max() < 1e-12
+
+
+def test_feature_acf_white_noise_and_box_blur():
+    """Statistics of known processes: white noise has ρ(t) = δ(t); a k×k box average of white
+    noise has the separable triangle ρ(dx, dy) = (1 − |dx|/k)(1 − |dy|/k) for |dx|, |dy| < k and
+    0 beyond (sampling error ~ 1/√(pixels/k²))."""
+    rng = np.random.default_rng(17)
+    noise = rng.standard_normal((4, 384, 384))
+    acf = O.feature_acf(noise)
+    off = np.delete(acf, _acf_index(0, 0), axis=0)
+    assert np.abs(acf[_acf_index(0, 0)] - 1).max() < 1e-12 and np.abs(off).max() < 0.02
+    k = 5
+    big = rng.standard_normal((4, 384 + k - 1, 384 + k - 1))
+    box = np.stack([np.lib.stride_tricks.sliding_window_view(b, (k, k)).mean(axis=(2, 3)) for b in big])
+    acf = O.feature_acf(box)
+    tri = lambda d: max(0.0, 1.0 - abs(d) / k)   # noqa: E731
+    want = np.array([tri(kk % 17 - 8) * tri(kk // 17 - 8) for kk in range(289)])
+    assert np.abs(acf - want[:, None]).max() < 0.05
+
+
+def test_corr_area_closed_forms():
+    """Bartlett area (1/16) Σ_t (Σ_i ρ_a,i(t)) (Σ_j ρ_b,j(t)), at least 1: white noise on both
+    sides (ρ = δ) gives exactly 1; two k×k-box-blurred sides (triangle ρ on all four planes) give
+    (Σ_{|d|<k} (1 − |d|/k)²)² — 3.4² = 11.56 for k = 5; one white side and one blurred side give
+    ρ_b(0) = 1, i.e. 1; and the area is symmetric in its sides."""
+    delta = np.zeros((289, 4))
+    delta[_acf_index(0, 0)] = 1.0
+    assert O.corr_area(delta, delta) == 1.0
+    k = 5
+    tri = lambda d: max(0.0, 1.0 - abs(d) / k)   # noqa: E731
+    t = np.array([tri(kk % 17 - 8) * tri(kk // 17 - 8) for kk in range(289)])
+    T = np.repeat(t[:, None], 4, axis=1)
+    line = sum(tri(d) ** 2 for d in range(-k + 1, k))
+    assert abs(line - 3.4) < 1e-12
+    assert abs(O.corr_area(T, T) - line ** 2) < 1e-9
+    assert O.corr_area(delta, T) == 1.0
+    rng = np.random.default_rng(3)
+    A, B = rng.uniform(0, 1, (289, 4)), rng.uniform(0, 1, (289, 4))
+    assert abs(O.corr_area(A, B) - O.corr_area(B, A)) < 1e-12
+    # measured end to end: box-blurred noise through feature_acf → corr_area near the closed form
+    big = rng.standard_normal((4, 384 + k - 1, 384 + k - 1))
+    box = np.stack([np.lib.stride_tricks.sliding_window_view(b, (k, k)).mean(axis=(2, 3)) for b in big])
+    acf = O.feature_acf(box)
+    assert abs(O.corr_area(acf, acf) / line ** 2 - 1) < 0.1

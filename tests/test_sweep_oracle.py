@@ -96,3 +96,60 @@ def test_sweep_recovers_exact_quarter_turn():
     assert np.allclose(top[0]["H"], truth, atol=1e-9)
     # best per candidate: θ-major; the true candidate is the landscape maximum
     assert int(np.argmax(r["best"])) == 9
+
+
+# ── the sweep's fractional-cell tap average (sweep.wgsl:64-85 tap, 106-145 pack_sweep) ──
+def _tap_points(cx, k, cell, o):
+    """0-based source coordinates of a cell's k taps along one axis under an identity pose: the
+    canvas point o + (c + (t + ½)/k)·cell (1-based pixels), minus 1."""
+    return o + (cx + (np.arange(k) + 0.5) / k) * cell - 1.0
+
+
+@pytest.mark.parametrize("k,cell,ox,oy", [(3, 1.37, -2.25, 3.5), (4, 2.6, 0.0, 0.0), (1, 0.83, 5.1, -1.7)])
+def test_sweep_window_constant_planes_count_inside_taps(k, cell, ox, oy):
+    """Constant feature planes: every tap whose four bilinear corners lie inside the source adds
+    weight 1 and the constant; the others add nothing. So a cell's weight plane is the fraction of
+    its k×k taps inside (0 ≤ x < w − 1, 0 ≤ y < h − 1 in 0-based coordinates) and each feature
+    plane that fraction times its constant, with the taps at (c + (t + ½)/k)·cell + o − 1."""
+    h, w = 19, 23
+    consts = np.array([0.5, -1.25, 2.0, 0.125])
+    feat = consts[:, None, None] * np.ones((4, h, w))
+    grid = 24
+    got = O.sweep_window(feat, np.eye(3), 0, 0, grid, grid, grid, k, ox, oy, cell)
+    for cy in range(grid):
+        ins_y = [(0 <= y < h - 1) for y in _tap_points(cy, k, cell, oy)]
+        for cx in range(grid):
+            ins_x = [(0 <= x < w - 1) for x in _tap_points(cx, k, cell, ox)]
+            frac = sum(ix and iy for iy in ins_y for ix in ins_x) / (k * k)
+            assert abs(got[0, cy, cx] - frac) < 1e-12, (cx, cy)
+            assert np.abs(got[1:, cy, cx] - consts * frac).max() < 1e-12, (cx, cy)
+    assert 0 < got[0].min() < 1 or got[0].min() == 0   # the window reaches past the source
+
+
+@pytest.mark.parametrize("k,cell", [(3, 1.37), (2, 0.9), (5, 2.2)])
+def test_sweep_window_reproduces_ramps_at_cell_centres(k, cell):
+    """Bilinear taps reproduce a linear function, and a cell's taps are symmetric about its
+    centre, so for a ramp f(x, y) = a + b x + c y (0-based source pixels) every cell whose taps are
+    all inside returns f at the image of the cell's centre: under an affine pose Hi (canvas →
+    source, 1-based) that is Hi · (o + (c + ½)·cell) − 1. A tap offset of t/k instead of
+    (t + ½)/k, a dropped 1-based shift or a transposed pose all move it."""
+    h, w = 60, 70
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    coef = [(0.3, 0.02, -0.05), (-1.0, 0.11, 0.07), (2.0, -0.03, 0.01), (0.0, 0.05, 0.05)]
+    feat = np.stack([a + b * x + c * y for a, b, c in coef])
+    t = np.deg2rad(11.0)
+    Hi = np.array([[0.97 * np.cos(t), -0.97 * np.sin(t), 9.0], [0.97 * np.sin(t), 0.97 * np.cos(t), 4.0],
+                   [0.0, 0.0, 1.0]])
+    ox, oy, grid = 1.5, -0.5, 40
+    got = O.sweep_window(feat, Hi, 0, 0, grid, grid, grid, k, ox, oy, cell)
+    checked = 0
+    for cy in range(grid):
+        for cx in range(grid):
+            if got[0, cy, cx] != 1.0:
+                continue   # a tap outside the source: not a full cell
+            px, py = ox + (cx + 0.5) * cell, oy + (cy + 0.5) * cell
+            sx, sy = Hi[0, 0] * px + Hi[0, 1] * py + Hi[0, 2] - 1, Hi[1, 0] * px + Hi[1, 1] * py + Hi[1, 2] - 1
+            for p, (a, b, c) in enumerate(coef):
+                assert abs(got[1 + p, cy, cx] - (a + b * sx + c * sy)) < 1e-9, (k, cell, cx, cy, p)
+            checked += 1
+    assert checked > 200

@@ -18,7 +18,7 @@ LIB = HERE / "libzcsmi.so"
 ARCH = os.environ.get("ZCSMI_ARCH", "gfx950")
 
 SOURCES = ["k_prep.hip", "k_pack.hip", "k_ifft.hip", "k_score.hip", "k_fft.hip", "k_pose.hip", "k_sweep.hip", "cnn.hip", "pre.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "rs.cpp", "climb.cpp", "abi.cpp"]
-HEADERS = ["kdev.hpp", "fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
+HEADERS = ["kdev.hpp", "events.hpp", "fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
            "search.hpp", "lie.hpp", "pre.hpp"]
 
 FLAGS = [
@@ -41,6 +41,22 @@ def _hipcc() -> str:
     raise FileNotFoundError("hipcc not found")
 
 
+def _deps_mtime(dfile: Path, fallback: float) -> float:
+    """Newest mtime of the headers a source included when it was last compiled (its -MMD file);
+    `fallback` (every header) when there is no dependency file."""
+    try:
+        text = dfile.read_text().replace("\\\n", " ")
+    except OSError:
+        return fallback
+    newest = 0.0
+    for tok in text.split(":", 1)[-1].split():
+        try:
+            newest = max(newest, Path(tok).stat().st_mtime)
+        except OSError:
+            return fallback   # a dependency went away: rebuild
+    return newest
+
+
 def build(force: bool = False, verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     hipcc = _hipcc()
@@ -50,11 +66,12 @@ def build(force: bool = False, verbose: bool = False) -> Path:
     for src in SOURCES:
         s = CSRC / src
         o = OBJ / (src + ".o")
+        d = OBJ / (src + ".d")
         objs.append(o)
-        if not force and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, hdr_mtime):
+        if not force and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, _deps_mtime(d, hdr_mtime)):
             continue
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [hipcc, *FLAGS, *PER_SOURCE.get(src, []), *lang, "-c", str(s), "-o", str(o)]
+        cmd = [hipcc, *FLAGS, *PER_SOURCE.get(src, []), *lang, "-MMD", "-MF", str(d), "-c", str(s), "-o", str(o)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

@@ -261,11 +261,11 @@ void zc_get_pose(void* p, double* H) {
     for (int i = 0; i < 9; ++i) H[i] = e->H[i];
 }
 
+// exports.zig:410-414: queued progress events (searches, climbs), whole JSON lines up to cap
+// bytes, removed from the queue; 0 when none are left (or the next line is longer than cap).
 size_t zc_events(void* p, char* buf, size_t cap) {
-  (void)p;
-  (void)buf;
-  (void)cap;
-  return 0;
+  Engine* e = static_cast<Engine*>(p);
+  return e ? e->events.take(buf, cap) : 0;
 }
 
 int32_t zc_profile(void* p, uint32_t on) {
@@ -539,8 +539,14 @@ int32_t zc_sweep_map(void* p, float* map, size_t len, uint32_t* dims) {
   return guard(e, "sweep_map", [&] { e->read_sweep_map(map, len, dims); });
 }
 
-// Candidates per sweep batch (0: 64).
+// Candidates per sweep batch (0: kSweepBatch, 256).
 void zc_sweep_tune(uint32_t batch) { zc::SweepTune::batch = (int)batch; }
+
+// Full-resolution maps at integer-translation poses from now on: 1 (default) through the
+// translation plans (both weight planes rectangles: 16 of 45 correlations by FFT), 0 through the
+// rectangle plans (30 by FFT). The two agree within FP32 rounding; process-wide.
+void zc_map_path(uint32_t translation_plans) { zc::MapPath::trans = translation_plans ? 1 : 0; }
+
 
 // ── reference entry points outside the SMI hot path: present so a ctypes binding of the
 //    reference's full symbol list loads, each failing with "Unsupported" ──

@@ -95,6 +95,7 @@ ClimbResult Engine::climb(uint32_t n_iters, bool hop, bool hop_fm, bool homograp
   };
 
   while (!done && !cancel) {
+    const uint32_t rec0 = n_rec;
     const Frame fr = reg_frame(Hc, A.w, A.h, B.w, B.h);
     const auto rs_c = moving_centroid(Hc, A.w, A.h);
     const bool mirror = Hc[0] * Hc[4] - Hc[1] * Hc[3] > 0;
@@ -139,7 +140,15 @@ ClimbResult Engine::climb(uint32_t n_iters, bool hop, bool hop_fm, bool homograp
       ++iter;
       if ((!moved && (!(hop || hop_fm) || stall >= 2)) || iter >= n_iters) done = true;
     }
+    // per batch (gclimb.zig:446-481): the pose it reached and where the climb stands
+    if (n_rec > rec0) {
+      const auto parts = score_parts(Hc, exact, symmetric);
+      events.trail("ascent", parts[0], parts[1], parts[2], Hc);
+      events.pose(Hc);
+    }
+    events.log("ascent %u/%u score %.4f  stall %u  (%u steps so far)", iter, n_iters, cur, stall, n_rec);
   }
+  events.log("ascent done  score %.4f", cur);
   H = Hc;
   ClimbResult r{};
   for (int i = 0; i < 9; ++i) r.H[i] = Hc[i];

@@ -522,7 +522,9 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     m.g = linear_corr_plan_rect(m.fr.w, m.fr.h, capn, ladder);
   }
   // the kernels take power-of-two lengths: a longer FFT only pads (a linear correlation's
-  // values per lag do not depend on the length once it is >= 2 cells - 1)
+  // values per lag do not depend on the length once it is >= 2 cells - 1); the map returned
+  // through the ABI is laid out on the reference plan's grid (read_map, fill_shift_result)
+  m.gr = m.g;
   m.g.nx = pow2_ceil(m.g.nx);
   m.g.ny = pow2_ceil(m.g.ny);
   if (m.g.nx > (uint32_t)kMaxFft || m.g.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
@@ -531,7 +533,14 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   // One canvas cell per pixel: the fixed weight plane is the indicator of the pasted rectangle, so
   // the correlations (a, 0) are rectangle sums (summed-area tables) instead of FFT correlations.
   const bool rect = cw == W && ch == Hh && W <= 2048;
-  const int plan = (exact ? 1 : 0) + (three ? 2 : 0) + (rect ? 4 : 0);
+  // An integer translation onto the canvas copies the moving stack exactly (every bilinear tap at a
+  // pixel centre), so its weight plane is the indicator of the moving rectangle as well: the
+  // correlations (0, b) become rectangle sums of the fixed side (translation plans).
+  const bool int_shift = Hc[0] == 1.0 && Hc[1] == 0.0 && Hc[3] == 0.0 && Hc[4] == 1.0 && Hc[6] == 0.0 &&
+                         Hc[7] == 0.0 && Hc[8] == 1.0 && Hc[2] == std::floor(Hc[2]) && Hc[5] == std::floor(Hc[5]) &&
+                         std::fabs(Hc[2]) < 1e6 && std::fabs(Hc[5]) < 1e6;
+  const bool trans = rect && int_shift && MapPath::trans != 0;
+  const int plan = (exact ? 1 : 0) + (three ? 2 : 0) + (rect ? 4 : 0) + (trans ? 4 : 0);
   const CorrPlan& pi = kPlans[plan];
   const int64_t N = (int64_t)nx * ny;
   const size_t cn = (size_t)W * Hh;
@@ -543,8 +552,8 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   float2* const spa = spec_a.as<float2>();
   float2* const spb = spa + N * pi.n_cplx;
   if (rect) {
-    sat_band.ensure((size_t)pi.n_sat * sat_bands(Hh) * W * 8);
-    sat.ensure((size_t)pi.n_sat * (W + 1) * (Hh + 1) * 8);
+    sat_band.ensure((size_t)std::max(pi.n_sat, pi.n_bsat) * sat_bands(Hh) * W * 8);
+    sat.ensure((size_t)(pi.n_sat + pi.n_bsat) * (W + 1) * (Hh + 1) * 8);
   }
   work.ensure((size_t)N * 8 * pi.n_out);
   corr.ensure((size_t)N * 4 * pi.n_corr);
@@ -562,9 +571,9 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st));
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st));
   LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
-  SatView sv{};
+  SatView sv{}, svb{};
   if (rect) {
-    LAUNCH("sat_build", ok &= launch_sat_build(plan, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), st));
+    LAUNCH("sat_build", ok &= launch_sat_build(plan, 0, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), st));
     sv.sat = sat.as<double>();
     sv.W = W;
     sv.H = Hh;
@@ -577,6 +586,21 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     sv.bx1 = std::min(W, -m.fr.ox + B.w);
     sv.by0 = std::max(0, -m.fr.oy);
     sv.by1 = std::min(Hh, -m.fr.oy + B.h);
+    sv.sign = 1;
+  }
+  if (trans) {
+    double* satb = sat.as<double>() + (int64_t)pi.n_sat * (W + 1) * (Hh + 1);
+    LAUNCH("sat_build", ok &= launch_sat_build(plan, 1, canvas_b.as<float>(), W, Hh, sat_band.as<double>(), satb, st));
+    svb = sv;
+    svb.sat = satb;
+    // the moving image copied to canvas cells [tx, tx + w) x [ty, ty + h); its tables are read over
+    // that rectangle moved by minus the lag
+    const int tx = (int)Hc[2], ty = (int)Hc[5];
+    svb.bx0 = std::min(std::max(0, tx), W);
+    svb.bx1 = std::min(std::max(0, tx + A.w), W);
+    svb.by0 = std::min(std::max(0, ty), Hh);
+    svb.by1 = std::min(std::max(0, ty + A.h), Hh);
+    svb.sign = -1;
   }
   LAUNCH("cmul", launch_cmul(plan, spa, spb, work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
@@ -591,7 +615,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   ca.ridge = kRidge;
   const double gcell = (double)W / (double)cw;
   ca.area = std::max(1.0, pair_area(A, B) / (gcell * gcell));
-  LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st, rect ? &sv : nullptr));
+  LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st, rect ? &sv : nullptr, trans ? &svb : nullptr));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
   prof.break_chain();
@@ -605,9 +629,17 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
 }
 
 void Engine::read_map(float* out, size_t len, const MapOut& m) {
-  const size_t nn = (size_t)m.g.nx * m.g.ny;
+  const size_t nn = (size_t)m.gr.nx * m.gr.ny;
   if (len < nn) throw std::runtime_error("MapTooSmall");
-  ZC_HIP(hipMemcpyAsync(out, map.p, nn * 4, hipMemcpyDeviceToHost, st));
+  const void* src = map.p;
+  if (m.gr.nx != m.g.nx || m.gr.ny != m.g.ny) {
+    // every lag of the reference grid, read from the (longer) grid the kernels ran on
+    map_out.ensure(nn * 4);
+    LAUNCH("map_remap", launch_map_remap(map.as<float>(), (int)m.g.nx, (int)m.g.ny, map_out.as<float>(),
+                                         (int)m.gr.nx, (int)m.gr.ny, st));
+    src = map_out.p;
+  }
+  ZC_HIP(hipMemcpyAsync(out, src, nn * 4, hipMemcpyDeviceToHost, st));
   prof.break_chain();
   ++counts[2];
   sync();
@@ -615,8 +647,8 @@ void Engine::read_map(float* out, size_t len, const MapOut& m) {
 
 void Engine::fill_shift_result(const MapOut& m, ShiftResult* r) const {
   std::memset(r, 0, sizeof *r);
-  r->n = m.g.nx;
-  r->n_y = m.g.ny;
+  r->n = m.gr.nx;
+  r->n_y = m.gr.ny;
   r->cw = m.g.cw;
   r->ch = m.g.ch;
   r->canvas_w = (uint32_t)m.fr.w;
@@ -636,7 +668,9 @@ void Engine::fill_shift_result(const MapOut& m, ShiftResult* r) const {
     const int64_t py = idx / m.g.nx, px = idx % m.g.nx;
     const int64_t lx = px <= m.g.nx / 2 ? px : px - (int64_t)m.g.nx;
     const int64_t ly = py <= m.g.ny / 2 ? py : py - (int64_t)m.g.ny;
-    r->peak_index = idx;
+    // the peak's index on the reference grid (its lag wrapped to that grid's lengths)
+    const int64_t jx = lx >= 0 ? lx : lx + (int64_t)m.gr.nx, jy = ly >= 0 ? ly : ly + (int64_t)m.gr.ny;
+    r->peak_index = (uint32_t)(jy * m.gr.nx + jx);
     r->dx = (double)lx * ((double)m.fr.w / (double)m.g.cw);
     r->dy = (double)ly * ((double)m.fr.h / (double)m.g.ch);
   }

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "events.hpp"
 #include "geometry.hpp"
 #include "kernels.hpp"
 #include "search.hpp"
@@ -210,7 +211,8 @@ struct GnOut {
 
 struct MapOut {
   Frame fr;
-  GridPlan g;
+  GridPlan g;    // the grid the kernels ran on (power-of-two FFT lengths)
+  GridPlan gr;   // the reference's plan (2^a·3^b lengths): the map and peak index the ABI returns
   bool exact, calib;
   float res[3];
 };
@@ -240,6 +242,7 @@ class Engine {
   std::string err;
   std::atomic<bool> cancel{false};
   Prof prof;
+  Events events;   // progress events (searches, climbs), drained by zc_events
   uint32_t counts[4] = {0, 0, 0, 0};   // dispatches, submits, reads, waits
   std::string device_name;
 
@@ -312,6 +315,8 @@ class Engine {
                     std::vector<std::array<double, 44>>& out);
   std::vector<double> pair_scores(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, bool exact,
                                   bool symmetric);
+  // the engine pair's score at H: mean, forward, inverse (NaN when not symmetric)
+  std::array<double, 3> score_parts(const Mat3& H, bool exact, bool symmetric);
   std::vector<GradOut> pair_grads(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, bool exact,
                                   bool homography, bool symmetric, bool hess);
   std::vector<GnOut> gn_batch(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, uint32_t iters,
@@ -347,6 +352,7 @@ class Engine {
   }
 
   DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
+  DBuf map_out;   // a map laid out on the reference plan's grid (read_map)
   DBuf sat_band, sat;   // rectangle plans: band sums and summed-area tables
   DBuf rs_fwd, rs_inv, rs_out, rs_tab;   // roto-scale maps: both directions, finished, log-polar tables
   DBuf pre_buf, pre_hist, pre_cdf, pre_part, pre_rgba;   // preprocessing scratch

@@ -266,6 +266,18 @@ std::vector<double> Engine::pair_scores(const std::vector<PairSides>& pr, const 
   return out;
 }
 
+std::array<double, 3> Engine::score_parts(const Mat3& Hp, bool exact, bool symmetric) {
+  std::vector<DirJob> d{{&sides[0], &sides[1], Hp}};
+  if (symmetric) d.push_back({&sides[1], &sides[0], inv3(Hp)});
+  std::vector<std::array<double, 45>> st;
+  moments_jobs(d, st);
+  auto fin = [](double v) { return std::isfinite(v) ? v : 0.0; };
+  const double f = fin(score_coefs(st[0].data(), exact, kRidge).score);
+  if (!symmetric) return {f, f, std::nan("")};
+  const double i = fin(score_coefs(st[1].data(), exact, kRidge).score);
+  return {0.5 * (f + i), f, i};
+}
+
 // Pair.gradBatch (pair.zig:276-306, pose.zig:267-289 gradFrom, pair.zig:241-268 sym).
 std::vector<GradOut> Engine::pair_grads(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, bool exact,
                                         bool homography, bool symmetric, bool hess) {
@@ -444,9 +456,40 @@ SearchResult Engine::search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, b
     *best_peak = 0;
     for (const SweepPeak& p : sw.peaks) *best_peak = std::max(*best_peak, p.score);
   }
-  const std::vector<SweepPeak> top = nms4(sw.peaks, (uint32_t)o.gv->thetas.size(), keep, o.gv->wrap);
+  const SweepGrid& gv = *o.gv;
+  const std::vector<SweepPeak> top = nms4(sw.peaks, (uint32_t)gv.thetas.size(), keep, gv.wrap);
   out.count_a = (uint32_t)top.size();
-  if (top.empty()) return out;
+  const double deg = 180.0 / M_PI;
+  events.log("search  %zuθ %.0f°…%.0f° × %zuσ %.3g–%.3g  %u maps  %u² grid (%.1f px, FFT %u²)  %s  %zu peaks",
+             gv.thetas.size(), gv.thetas.front() * deg, gv.thetas.back() * deg, gv.sigmas.size(), gv.sigmas.front(),
+             gv.sigmas.back(), sw.nc, sw.grid, sw.cell, sw.n, exact ? "SMI" : "global SMI", top.size());
+  // the sweep's landscape (search.zig:822-833 emitSweep): each (θ, σ) cell's best score over
+  // shears and stretches, the NMS peaks, finalists, the result and the climbed candidates
+  const size_t nsh = gv.shears.size() * gv.anisos.size();
+  std::vector<double> land(gv.thetas.size() * gv.sigmas.size(), 0.0);
+  for (size_t c = 0; c < land.size() && c * nsh < sw.best.size(); ++c)
+    for (size_t k = 0; k < nsh && c * nsh + k < sw.best.size(); ++k) land[c] = std::max(land[c], (double)sw.best[c * nsh + k]);
+  auto sweep_event = [&](const std::vector<size_t>& fin, const Mat3* result, const std::string& cands, bool applied) {
+    std::string tops = "[", fins = "[";
+    for (size_t i = 0; i < top.size(); ++i)
+      tops += (i ? ",[" : "[") + std::to_string(top[i].it) + "," + std::to_string(top[i].is) + "]";
+    for (size_t i = 0; i < fin.size(); ++i) fins += (i ? "," : "") + std::to_string(fin[i]);
+    JsonObj ev;
+    ev.str("kind", "sweep").nums("thetas", gv.thetas).nums("sigmas", gv.sigmas).boolean("wrap", gv.wrap);
+    ev.nums("best", land).raw("top", tops + "]").raw("finalists", fins + "]");
+    if (result) {
+      const auto ts = sim_of(*result, o.H0);
+      ev.nums("result", ts.data(), 2);
+    } else {
+      ev.null("result");
+    }
+    events.emit(ev.raw("cands", cands).integer("applied", applied ? 0 : -1));
+  };
+  if (top.empty()) {
+    events.log("search  no peak above the overlap floor");
+    sweep_event({}, nullptr, "[]", false);
+    return out;
+  }
   std::vector<Mat3> starts;
   for (const SweepPeak& p : top) starts.push_back(p.H);
   const std::vector<GnOut> screened = gn_batch(own_pair(starts.size()), starts, 4, homography, exact, symmetric, amps);
@@ -459,6 +502,12 @@ SearchResult Engine::search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, b
     if (std::find(fin.begin(), fin.end(), order[k]) == fin.end()) fin.push_back(order[k]);
   for (size_t k = 0; k < std::min<size_t>(n_fin, top.size()); ++k)
     if (std::find(fin.begin(), fin.end(), k) == fin.end()) fin.push_back(k);
+  {
+    std::string f;
+    for (size_t i = 0; i < fin.size(); ++i) f += (i ? " #" : "#") + std::to_string(fin[i] + 1);
+    events.log("search  screen %zu × 4 steps  best nS %.1f (peak #%zu)  finalists %s", top.size(),
+               screened[order[0]].score, order[0] + 1, f.c_str());
+  }
   std::vector<Mat3> fh;
   for (size_t k : fin) fh.push_back(screened[k].H);
   const std::vector<GnOut> finals = gn_batch(own_pair(fh.size()), fh, 25, homography, exact, symmetric, amps);
@@ -466,11 +515,42 @@ SearchResult Engine::search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, b
   for (size_t i = 0; i < finals.size(); ++i)
     if (finals[i].score > finals[bi].score) bi = i;
   out.count_b = (uint32_t)finals.size();
-  if (beats(finals[bi].score, s_cur)) {
+  {
+    std::string f;
+    char b[32];
+    for (size_t i = 0; i < finals.size(); ++i) {
+      std::snprintf(b, sizeof b, "%s%.1f", i ? " / " : "", finals[i].score);
+      f += b;
+    }
+    events.log("search  climb %zu × ≤25 GN  nS %s", finals.size(), f.c_str());
+  }
+  // the climbed finalists, best first
+  std::vector<size_t> rank(finals.size());
+  std::iota(rank.begin(), rank.end(), 0);
+  std::stable_sort(rank.begin(), rank.end(), [&](size_t a, size_t b) { return finals[a].score > finals[b].score; });
+  std::string cands = "[";
+  for (size_t r = 0; r < rank.size(); ++r) {
+    const size_t i = rank[r];
+    const auto ts = sim_of(finals[i].H, o.H0);
+    JsonObj c;
+    c.mat("H", finals[i].H).num("score", finals[i].score).integer("peak", (long long)fin[i] + 1);
+    cands += (r ? "," : "") + c.num("th", ts[0]).num("s", ts[1]).done();
+  }
+  const bool improved = beats(finals[bi].score, s_cur);
+  sweep_event(fin, &finals[bi].H, cands + "]", improved);
+  if (improved) {
     H = finals[bi].H;
     for (int i = 0; i < 9; ++i) out.H[i] = H[i];
     out.score = finals[bi].score;
     out.improved = 1;
+    const auto parts = score_parts(H, exact, symmetric);
+    events.trail("search", parts[0], parts[1], parts[2], H);
+    events.pose(H);
+    const auto ts = sim_of(H, o.H0);
+    events.log("search done  nS %.1f (was %.1f)  θ %.1f° σ %.3f  peak #%zu", out.score, s_cur, ts[0] * deg, ts[1],
+               fin[bi] + 1);
+  } else {
+    events.log("search done  kept the current pose (nS %.1f ≥ %.1f)", s_cur, finals[bi].score);
   }
   return out;
 }

@@ -6,26 +6,6 @@ namespace zc {
 // ── row IFFT + extract ────────────────────────────────────────────────────────────────────
 // Inverse row FFT of complex output q (rows of L) → correlation planes 2q (re) and 2q+1 (im).
 // blockIdx.y = b · n_out + q for batch entry b (its planes at b · n_corr).
-// Row tilings of the inverse row pass (zc_fft_tune's rows value) for lines of >= 256 points:
-// 0 (default) 8 points per lane at 256 (0.287 -> 0.270 ms for a 128²-canvas sweep batch), two
-// 1024-point lines per workgroup at 16 points per lane, 32 points per lane from 2048 (0.237 ->
-// 0.225 ms for the 15 planes of a 2048² map, 1.54 -> 1.33 ms for a sweep batch's 4096-point
-// planes), RowT's tiling otherwise; 1 32 points per lane; 2 8 points
-// per lane; 3 16 points per lane with RowT's lines per workgroup; 4 16 points per lane, twice
-// RowT's lines. Every workgroup holds at least one wave. Shorter lines always take RowT's
-// tiling. Lines whose radix-16/8 stages split the same way at 16 and 32 points per lane (2048,
-// 4096) give the same results bit for bit at either.
-template <int L, int V>
-struct RowX {
-  static constexpr bool WIDE = V == 1 || (V == 0 && L >= 2048);
-  static constexpr int M = WIDE ? 32 : (V == 2 || (V == 0 && L == 256)) ? 8 : 16;
-  static constexpr int TWO = (V == 4 || (V == 0 && L == 1024)) ? 2 : 1;
-  static constexpr int MIN_LINES = (64 * M + L - 1) / L;
-  static constexpr int LINES =
-      RowT<L>::LINES * TWO > MIN_LINES ? RowT<L>::LINES * TWO : MIN_LINES;
-  static constexpr int T = LINES * L / M;
-  static constexpr int STR = RowT<L>::STR;
-};
 
 template <int L, int PID, int V>
 __global__ __launch_bounds__((RowX<L, V>::T)) void k_rowifft_extract(const float2* __restrict__ work, float* corr,
@@ -70,7 +50,11 @@ static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const 
     case 4: k_rowifft_extract<L, 4, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
     case 5: k_rowifft_extract<L, 5, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
     case 6: k_rowifft_extract<L, 6, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    default: k_rowifft_extract<L, 7, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 7: k_rowifft_extract<L, 7, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 8: k_rowifft_extract<L, 8, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 9: k_rowifft_extract<L, 9, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 10: k_rowifft_extract<L, 10, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    default: k_rowifft_extract<L, 11, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
   }
 }
 

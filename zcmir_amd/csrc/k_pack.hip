@@ -208,7 +208,15 @@ static void pack_rowfft_L(int plan, int side, const float* canvas, const Resampl
     case 12: K(12); break;
     case 13: K(13); break;
     case 14: K(14); break;
-    default: K(15); break;
+    case 15: K(15); break;
+    case 16: K(16); break;
+    case 17: K(17); break;
+    case 18: K(18); break;
+    case 19: K(19); break;
+    case 20: K(20); break;
+    case 21: K(21); break;
+    case 22: K(22); break;
+    default: K(23); break;
   }
 #undef K
 }
@@ -271,7 +279,11 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
     case 4: k_cmul<4><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
     case 5: k_cmul<5><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
     case 6: k_cmul<6><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    default: k_cmul<7><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 7: k_cmul<7><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 8: k_cmul<8><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 9: k_cmul<9><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 10: k_cmul<10><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    default: k_cmul<11><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
   }
 }
 

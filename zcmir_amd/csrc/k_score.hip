@@ -74,10 +74,10 @@ __device__ __forceinline__ void canvas_values(const float* __restrict__ canvas, 
 
 // blockIdx.x band, blockIdx.y a chunk of 256 columns: band[(k · nb + b) · W + x] = column sums of
 // table k's plane over the band's rows, for every table.
-template <int PID>
+template <int SID>
 __global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canvas, int W, int H, double* band) {
-  constexpr CorrPlan P = kPlans[PID];
-  constexpr int n_tab = P.n_sat;
+  constexpr SatSet P = sat_set(SID);
+  constexpr int n_tab = P.n;
   const int b = blockIdx.x, nb = gridDim.x;
   const int x = blockIdx.y * 256 + threadIdx.x;
   if (x >= W) return;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_sat_band(const float* __restrict__ canv
       const double inv0 = canvas_inv0(sv[i]);
 #pragma unroll
       for (int k = 0; k < 15; ++k)
-        if (k < n_tab) acc[k] += real_plane_d(P.sat_plane[k], sv[i], inv0);
+        if (k < n_tab) acc[k] += real_plane_d(P.plane[k], sv[i], inv0);
     }
   }
 #pragma unroll
@@ -163,17 +163,17 @@ __device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
 // blockIdx.x band, blockIdx.y a group of G tables (one read of the canvas for all of them);
 // thread t owns columns [t·E, t·E + E) for the scans; each finished row goes out through LDS so
 // that lane i stores column x + i (whole cache lines per store instruction).
-template <int PID, int NT, int E, int G, int K0>
+template <int SID, int NT, int E, int G, int K0>
 __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, int W, int H,
                                               const double* __restrict__ band, double* sat, double* lds,
                                               double (*stage)[NT * E]) {
-  constexpr CorrPlan P = kPlans[PID];
-  constexpr int n_tab = P.n_sat, k0 = K0;
+  constexpr SatSet P = sat_set(SID);
+  constexpr int n_tab = P.n, k0 = K0;
   const int b = blockIdx.x, nb = gridDim.x;
   const int xs = threadIdx.x * E;
   const int64_t n = (int64_t)W * H, SW = W + 1, SP = SW * (H + 1);
-  constexpr int pl[G] = {P.sat_plane[k0 < n_tab ? k0 : 0], P.sat_plane[k0 + 1 < n_tab ? k0 + 1 : 0],
-                         P.sat_plane[k0 + 2 < n_tab ? k0 + 2 : 0]};
+  constexpr int pl[G] = {P.plane[k0 < n_tab ? k0 : 0], P.plane[k0 + 1 < n_tab ? k0 + 1 : 0],
+                         P.plane[k0 + 2 < n_tab ? k0 + 2 : 0]};
   static_assert(G == 3, "three tables per workgroup");
   // column carry cc[g][e] = S_g(y, xs + e + 1): the row prefix of the earlier bands' column sums
   double cc[G][E];
@@ -256,91 +256,93 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
 
 // blockIdx.y: the group of G tables (tables G·y .. G·y + G − 1 of the plan); NT threads of E
 // columns each cover rows of up to NT·E cells.
-template <int PID, int NT, int E, int G>
+template <int SID, int NT, int E, int G>
 __global__ __launch_bounds__(NT) void k_sat_rows(const float* __restrict__ canvas, int W, int H,
                                                  const double* __restrict__ band, double* sat) {
   __shared__ double lds[NT / 64 * G];
   __shared__ double stage[G][NT * E];
-  static_assert(kPlans[PID].n_sat <= 5 * G, "at most five groups");
+  static_assert(sat_set(SID).n <= 5 * G, "at most five groups");
   switch (blockIdx.y) {
-    case 0: sat_rows_body<PID, NT, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
-    case 1: sat_rows_body<PID, NT, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
-    case 2: sat_rows_body<PID, NT, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
-    case 3: sat_rows_body<PID, NT, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
-    default: sat_rows_body<PID, NT, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
+    case 0: sat_rows_body<SID, NT, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
+    case 1: sat_rows_body<SID, NT, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
+    case 2: sat_rows_body<SID, NT, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
+    case 3: sat_rows_body<SID, NT, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
+    default: sat_rows_body<SID, NT, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
   }
 }
 
 int sat_bands(int H) { return (H + kSatBand - 1) / kSatBand; }
 
-template <int PID>
-static void sat_build_P(const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
-  constexpr int n_tab = kPlans[PID].n_sat;
+template <int SID>
+static void sat_build_S(const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+  constexpr int n_tab = sat_set(SID).n;
   const int nb = sat_bands(H);
-  k_sat_band<PID><<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, band);
+  k_sat_band<SID><<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, band);
   k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 64 * kCarryWaves, 0, st>>>(band, W, nb);
   constexpr int G = 3;   // tables per workgroup
   const dim3 g(nb, (n_tab + G - 1) / G);
-  if (W <= 256 * 4) k_sat_rows<PID, 256, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
-  else k_sat_rows<PID, 512, 4, G><<<g, 512, 0, st>>>(canvas, W, H, band, sat);
+  if (W <= 256 * 4) k_sat_rows<SID, 256, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
+  else k_sat_rows<SID, 512, 4, G><<<g, 512, 0, st>>>(canvas, W, H, band, sat);
 }
 
-bool launch_sat_build(int plan, const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, double* band, double* sat,
+                      hipStream_t st) {
   if (W < 1 || H < 1 || W > 256 * 8) return false;
-  switch (plan) {
-    case 4: sat_build_P<4>(canvas, W, H, band, sat, st); return true;
-    case 5: sat_build_P<5>(canvas, W, H, band, sat, st); return true;
-    case 6: sat_build_P<6>(canvas, W, H, band, sat, st); return true;
-    case 7: sat_build_P<7>(canvas, W, H, band, sat, st); return true;
+  // side A of a translation plan sums the same planes as its rectangle plan (plan − 4)
+  const int sid = side == 0 ? (plan >= 8 ? plan - 4 : plan) : 16 + plan;
+  switch (sid) {
+    case 4: sat_build_S<4>(canvas, W, H, band, sat, st); return true;
+    case 5: sat_build_S<5>(canvas, W, H, band, sat, st); return true;
+    case 6: sat_build_S<6>(canvas, W, H, band, sat, st); return true;
+    case 7: sat_build_S<7>(canvas, W, H, band, sat, st); return true;
+    case 24: sat_build_S<24>(canvas, W, H, band, sat, st); return true;
+    case 25: sat_build_S<25>(canvas, W, H, band, sat, st); return true;
+    case 26: sat_build_S<26>(canvas, W, H, band, sat, st); return true;
+    case 27: sat_build_S<27>(canvas, W, H, band, sat, st); return true;
     default: return false;
   }
 }
 
-// Table offsets of the rectangle R + (dx, dy) clipped to the canvas for lag index t.
+// Table offsets (elements) of the rectangle R ± (dx, dy) clipped to the canvas for lag index t:
+// S(y0, x0), S(y0, x1), S(y1, x0), S(y1, x1). The clamps keep x0 <= x1 and y0 <= y1, so an empty
+// rectangle sums to exactly 0, and the tables' row 0 and column 0 hold zeros, so every corner is
+// read unconditionally (no branches; the zero row and column stay in the caches).
 struct SatIdx {
-  int64_t o00, o01, o10, o11;   // S(y0, x0), S(y0, x1), S(y1, x0), S(y1, x1)
-  bool empty;
-  bool r0, c0;                  // y0 = 0 / x0 = 0: those corners lie on the table's zero row / column
+  uint32_t o00, o01, o10, o11;
 };
 
 __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
   const int px = (int)(t & (v.nx - 1)), py = (int)(t >> v.lx);
   const int dx = px <= v.nx / 2 ? px : px - v.nx;
   const int dy = py <= v.ny / 2 ? py : py - v.ny;
-  const int x0 = min(max(v.bx0 + dx, 0), v.W), x1 = min(max(v.bx1 + dx, 0), v.W);
-  const int y0 = min(max(v.by0 + dy, 0), v.H), y1 = min(max(v.by1 + dy, 0), v.H);
-  SatIdx q;
-  const int64_t SW = v.W + 1;
-  q.empty = x1 <= x0 || y1 <= y0;
-  q.r0 = y0 == 0;
-  q.c0 = x0 == 0;
-  q.o00 = y0 * SW + x0;
-  q.o01 = y0 * SW + x1;
-  q.o10 = y1 * SW + x0;
-  q.o11 = y1 * SW + x1;
-  return q;
+  const int sx = v.sign * dx, sy = v.sign * dy;
+  const int x0 = min(max(v.bx0 + sx, 0), v.W), x1 = min(max(v.bx1 + sx, 0), v.W);
+  const int y0 = min(max(v.by0 + sy, 0), v.H), y1 = min(max(v.by1 + sy, 0), v.H);
+  const uint32_t SW = (uint32_t)v.W + 1;
+  return {y0 * SW + x0, y0 * SW + x1, y1 * SW + x0, y1 * SW + x1};
 }
 
-// Corners on the zero row or column are not read (half the lags of a map have y0 = 0 or x0 = 0).
-__device__ __forceinline__ double sat_sum(const double* __restrict__ S, const SatIdx& q) {
-  if (q.empty) return 0.0;
-  const double s11 = S[q.o11];
-  const double s01 = q.r0 ? 0.0 : S[q.o01];
-  const double s10 = q.c0 ? 0.0 : S[q.o10];
-  const double s00 = (q.r0 || q.c0) ? 0.0 : S[q.o00];
-  return (s11 - s01) - (s10 - s00);
+// The sum over the rectangle from table k (base = k · table size, in elements; every table set
+// of a map stays below 2^32 bytes, so 32-bit element offsets address it).
+__device__ __forceinline__ double sat_sum(const double* __restrict__ S, uint32_t base, const SatIdx& q) {
+  return (S[base + q.o11] - S[base + q.o01]) - (S[base + q.o10] - S[base + q.o00]);
 }
 
-// k_combine for the rectangle plans: correlations with sat_of >= 0 come from the tables.
+// k_combine for the rectangle plans: correlations with sat_of >= 0 come from side A's tables, and
+// in translation plans those with bsat_of >= 0 from side B's.
 template <int PID>
 __global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ corr, float* map, int64_t nn,
-                                                      CombineArgs a, SatView sv, float* arg_part) {
+                                                      CombineArgs a, SatView sv, SatView svb, float* arg_part) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr bool EXACT = PID & 1;
-  const int64_t SP = (int64_t)(sv.W + 1) * (sv.H + 1);
-  SatIdx q = sat_idx(sv, 0);
+  constexpr bool TRANS = P.n_bsat > 0;
+  const uint32_t SP = (uint32_t)(sv.W + 1) * (uint32_t)(sv.H + 1);
+  SatIdx q = sat_idx(sv, 0), qb = q;
+  if constexpr (TRANS) qb = sat_idx(svb, 0);
   auto C = [&](int num, int64_t t) -> double {
-    if (P.sat_of[num] >= 0) return sat_sum(sv.sat + P.sat_of[num] * SP, q);
+    if (P.sat_of[num] >= 0) return sat_sum(sv.sat, (uint32_t)P.sat_of[num] * SP, q);
+    if constexpr (TRANS)
+      if (P.bsat_of[num] >= 0) return sat_sum(svb.sat, (uint32_t)P.bsat_of[num] * SP, qb);
     return (double)corr[(int64_t)P.kept_of[num] * nn + t];
   };
   const double n0 = C(0, 0);
@@ -350,6 +352,7 @@ __global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ 
   uint32_t bi = 0xffffffffu;
   for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
     q = sat_idx(sv, t);
+    if constexpr (TRANS) qb = sat_idx(svb, t);
     const float s = (float)lag_score<PID & 3>(C, t, floor_n, a, calib);
     map[t] = s;
     if (better(s, (uint32_t)t, bv, bi)) {
@@ -490,19 +493,41 @@ __global__ __launch_bounds__(256) void k_argmax_finish(const float* __restrict__
 int combine_blocks(int64_t nn) { return (int)std::min<int64_t>((nn + 255) / 256, 4096); }
 
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
-                    hipStream_t st, const SatView* sv) {
+                    hipStream_t st, const SatView* sv, const SatView* svb) {
   const int g = combine_blocks(nn);
   if (plan >= 4 && !sv) throw std::runtime_error("rectangle plan without tables");
+  if (plan >= 8 && !svb) throw std::runtime_error("translation plan without the fixed side's tables");
+  const SatView b = svb ? *svb : SatView{};
   switch (plan) {
     case 0: k_combine<0><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 1: k_combine<1><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 2: k_combine<2><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 3: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
-    case 4: k_combine_rect<4><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
-    case 5: k_combine_rect<5><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
-    case 6: k_combine_rect<6><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
-    default: k_combine_rect<7><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, arg_part); break;
+    case 4: k_combine_rect<4><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 5: k_combine_rect<5><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 6: k_combine_rect<6><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 7: k_combine_rect<7><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 8: k_combine_rect<8><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 9: k_combine_rect<9><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 10: k_combine_rect<10><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    default: k_combine_rect<11><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
   }
+}
+
+__global__ __launch_bounds__(256) void k_map_remap(const float* __restrict__ in, int nx, int ny, float* out, int rx,
+                                                   int ry) {
+  const int64_t n = (int64_t)rx * ry;
+  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int jy = (int)(t / rx), jx = (int)(t - (int64_t)jy * rx);
+    const int ix = 2 * jx <= rx ? jx : jx - rx + nx;
+    const int iy = 2 * jy <= ry ? jy : jy - ry + ny;
+    out[t] = in[(int64_t)iy * nx + ix];
+  }
+}
+
+void launch_map_remap(const float* in, int nx, int ny, float* out, int rx, int ry, hipStream_t st) {
+  if (rx > nx || ry > ny || rx <= 0 || ry <= 0) throw std::runtime_error("map_remap: the grid to read is smaller");
+  k_map_remap<<<(unsigned)std::min<int64_t>(((int64_t)rx * ry + 255) / 256, 8192), 256, 0, st>>>(in, nx, ny, out, rx, ry);
 }
 
 void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st) {

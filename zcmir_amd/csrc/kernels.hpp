@@ -61,6 +61,11 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
                  hipStream_t st, int batch = 1);
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
                             hipStream_t st, int batch = 1);
+// Maps at integer-translation poses (full resolution): 1 (default) the translation plans, 0 the
+// rectangle plans every other full-resolution pose takes; process-wide (zc_map_path)
+struct MapPath {
+  static inline std::atomic<int> trans{1};
+};
 struct CombineArgs {
   int exact, calib, nf;
   float min_n;
@@ -71,14 +76,23 @@ int combine_blocks(int64_t nn);
 // kPlans[plan].sat_plane[k].
 int sat_bands(int H);
 // canvas: 5 planes W x H; band: n_sat · sat_bands(H) · W doubles; sat: n_sat · (W+1)(H+1) doubles
-bool launch_sat_build(int plan, const float* canvas, int W, int H, double* band, double* sat, hipStream_t st);
+// side 0: side A's tables of a rectangle or translation plan; side 1: side B's (translation plans)
+bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, double* band, double* sat,
+                      hipStream_t st);
 // The tables and the fixed rectangle [bx0, bx1) x [by0, by1) in canvas cells; nx = 1 << lx.
+// sign +1: the rectangle moves with the lag (side A's tables, R = the fixed rectangle: R + t);
+// -1: against it (side B's tables of a translation plan, R = the moving rectangle: R − t).
 struct SatView {
   const double* sat;
   int W, H, nx, ny, lx, bx0, bx1, by0, by1;
+  int sign;
 };
+// sv: side A's tables (rectangle plans); svb: side B's (translation plans)
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
-                    hipStream_t st, const SatView* sv = nullptr);
+                    hipStream_t st, const SatView* sv = nullptr, const SatView* svb = nullptr);
+// out (ry x rx) = the lags of a wrapped map (ny x nx, nx >= rx, ny >= ry) on a grid of rx x ry:
+// column j holds lag j for 2j <= rx, else lag j - rx (rows likewise)
+void launch_map_remap(const float* in, int nx, int ny, float* out, int rx, int ry, hipStream_t st);
 void launch_argmax_finish(const float* arg_part, int nparts, const float* map, float* res, hipStream_t st);
 
 // ── roto-scale maps ──

@@ -14,6 +14,12 @@
 // the lag, read from a summed-area table of plane a in O(1) per lag. Those correlations leave the
 // FFT path; the moving side then transforms only the planes of the remaining correlations (w and
 // the features) and the fixed side only its planes other than w.
+//
+// Translation plans (`trans`, rectangle plans whose moving stack lands on the canvas by an integer
+// translation): the moving weight plane is then the indicator of a rectangle too, so every
+// correlation (0, b) is the sum of fixed plane b over the moving rectangle shifted by minus the
+// lag — a read of the fixed side's tables. Only the cross terms (i, j) of the features stay in the
+// FFT path: 16 correlations of 4 + 4 planes (9 of 3 + 3 with three features).
 #pragma once
 
 namespace zc {
@@ -33,9 +39,12 @@ struct CorrPlan {
   int n_sat;                // rectangle-sum planes of side A
   int sat_plane[15];        // rectangle-sum index → plane of A
   int sat_of[45];           // number → rectangle-sum index, -1 if not one
+  int n_bsat;               // translation plans: rectangle-sum planes of side B
+  int bsat_plane[15];       // B rectangle-sum index → plane of B
+  int bsat_of[45];          // number → B rectangle-sum index, -1 if not one
 };
 
-constexpr CorrPlan make_plan(bool exact, bool three, bool rect) {
+constexpr CorrPlan make_plan(bool exact, bool three, bool rect, bool trans = false) {
   CorrPlan p{};
   int la[45] = {}, lb[45] = {};
   int len = 0;
@@ -59,9 +68,10 @@ constexpr CorrPlan make_plan(bool exact, bool three, bool rect) {
   for (int c = 0; c < 45; ++c) {
     p.kept_of[c] = -1;
     p.sat_of[c] = -1;
+    p.bsat_of[c] = -1;
   }
-  int sat_idx[15];
-  for (int pl = 0; pl < 15; ++pl) sat_idx[pl] = -1;
+  int sat_idx[15] = {}, bsat_idx[15] = {};
+  for (int pl = 0; pl < 15; ++pl) sat_idx[pl] = bsat_idx[pl] = -1;
   for (int c = 0; c < len; ++c) {
     if (!valid[la[c]] || !valid[lb[c]]) continue;
     if (rect && lb[c] == 0) {
@@ -70,6 +80,12 @@ constexpr CorrPlan make_plan(bool exact, bool three, bool rect) {
         p.sat_plane[p.n_sat++] = la[c];
       }
       p.sat_of[c] = sat_idx[la[c]];
+    } else if (trans && la[c] == 0) {
+      if (bsat_idx[lb[c]] < 0) {
+        bsat_idx[lb[c]] = p.n_bsat;
+        p.bsat_plane[p.n_bsat++] = lb[c];
+      }
+      p.bsat_of[c] = bsat_idx[lb[c]];
     } else {
       use_a[la[c]] = true;
       use_b[lb[c]] = true;
@@ -93,7 +109,7 @@ constexpr CorrPlan make_plan(bool exact, bool three, bool rect) {
   p.b_n_cplx = (p.b_n_real + 1) / 2;
   int kept = 0;
   for (int c = 0; c < len; ++c) {
-    if (!valid[la[c]] || !valid[lb[c]] || p.sat_of[c] >= 0) continue;
+    if (!valid[la[c]] || !valid[lb[c]] || p.sat_of[c] >= 0 || p.bsat_of[c] >= 0) continue;
     p.ca[kept] = p.plane_slot[la[c]];
     p.cb[kept] = p.b_plane_slot[lb[c]];
     p.cnum[kept] = c;
@@ -105,11 +121,13 @@ constexpr CorrPlan make_plan(bool exact, bool three, bool rect) {
   return p;
 }
 
-// plan id: exact + 2 · three + 4 · rect
-constexpr CorrPlan kPlans[8] = {make_plan(false, false, false), make_plan(true, false, false),
-                                make_plan(false, true, false),  make_plan(true, true, false),
-                                make_plan(false, false, true),  make_plan(true, false, true),
-                                make_plan(false, true, true),   make_plan(true, true, true)};
+// plan id: exact + 2 · three + 4 · rect + 4 · trans (translation plans are rectangle plans: 8..11)
+constexpr CorrPlan kPlans[12] = {make_plan(false, false, false), make_plan(true, false, false),
+                                 make_plan(false, true, false),  make_plan(true, true, false),
+                                 make_plan(false, false, true),  make_plan(true, false, true),
+                                 make_plan(false, true, true),   make_plan(true, true, true),
+                                 make_plan(false, false, true, true), make_plan(true, false, true, true),
+                                 make_plan(false, true, true, true),  make_plan(true, true, true, true)};
 
 // Without rectangle sums both sides transform the same planes.
 static_assert(kPlans[1].n_real == 15 && kPlans[1].b_n_real == 15 && kPlans[1].n_corr == 45, "exact plan");
@@ -119,6 +137,26 @@ static_assert(kPlans[5].n_sat == 15 && kPlans[5].n_real == 5 && kPlans[5].b_n_re
 static_assert(kPlans[7].n_sat == 10 && kPlans[7].n_real == 4 && kPlans[7].b_n_real == 9 && kPlans[7].n_corr == 18,
               "three-feature rectangle plan");
 static_assert(kPlans[4].n_sat == 5 && kPlans[4].n_corr == 20, "global rectangle plan");
+static_assert(kPlans[9].n_sat == 15 && kPlans[9].n_bsat == 14 && kPlans[9].n_corr == 16 && kPlans[9].n_real == 4 &&
+                  kPlans[9].b_n_real == 4,
+              "exact translation plan");
+static_assert(kPlans[11].n_sat == 10 && kPlans[11].n_bsat == 9 && kPlans[11].n_corr == 9 && kPlans[11].n_real == 3,
+              "three-feature translation plan");
+static_assert(kPlans[8].n_sat == 5 && kPlans[8].n_bsat == 4 && kPlans[8].n_corr == 16, "global translation plan");
+
+// The planes of one side's summed-area tables: set s < 16 is side A of plan s, 16 + s side B.
+struct SatSet {
+  int n;
+  int plane[15];
+};
+
+constexpr SatSet sat_set(int sid) {
+  SatSet t{};
+  const CorrPlan& p = kPlans[sid & 15];
+  t.n = sid < 16 ? p.n_sat : p.n_bsat;
+  for (int k = 0; k < 15; ++k) t.plane[k] = sid < 16 ? p.sat_plane[k] : p.bsat_plane[k];
+  return t;
+}
 
 // The side tables the pack kernels use: index 2 · plan + side (side B of a plan without
 // rectangle sums shares side A's table).

@@ -100,6 +100,14 @@ inline Mat3 sim_about(const std::array<double, 2>& c, double th, double sg) {
 
 // One raw peak of the sweep (search.zig:145): score (n in fixed pixels), candidate and grid
 // indices, the candidate's θ and σ, the canvas shift in cells and the corrected pose.
+// Rotation (radians) and scale of pose H relative to H0: the linear part of H · H0⁻¹ (the
+// search's result as the sweep reports it; search.zig simOf's role).
+inline std::array<double, 2> sim_of(const Mat3& H, const Mat3& H0) {
+  const Mat3 M = mul3(H, inv3(H0));
+  const double a = M[0] / M[8], b = M[1] / M[8], c = M[3] / M[8], d = M[4] / M[8];
+  return {std::atan2(c - b, a + d), std::sqrt(std::fabs(a * d - b * c))};
+}
+
 struct SweepPeak {
   double score;
   uint32_t cand, it, is, ik, ia;

@@ -331,12 +331,11 @@ def sweep_labels(reg, moving, fixed, grid=64):
     torch.cuda.synchronize()
     reg.configure(sw_nth=N_TH, sw_smin=0.8, sw_smax=1.25, sw_ns=N_SG, sw_grid=grid, sw_rot_range=False, sw_rot=180,
                   sw_full=False)
-    lib = reg._lib
     labels, best_norm = [], []
     n, h, w = fixed.shape
     for i in range(n):
-        reg._check(lib.zc_set_image_device(reg._e, 0, _p(moving[i]), w, h), "set_image_device")
-        reg._check(lib.zc_set_image_device(reg._e, 1, _p(fixed[i]), w, h), "set_image_device")
+        reg.set_image_device(0, _p(moving[i]), w, h)
+        reg.set_image_device(1, _p(fixed[i]), w, h)
         _, _, best = reg.sweep()
         k = int(np.argmax(best))
         labels.append(k)
