@@ -123,11 +123,13 @@ def bench_map2048(a, np, torch, zcmir_amd, synth):
         raise SystemExit(f"2048 map workload check failed: peak ({sm.dx}, {sm.dy}), grid {sm.n}x{sm.ny}")
     reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=2)
     torch.cuda.synchronize()
-    reg.profile(True)
     t0 = time.perf_counter()
     reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=a.map2048_steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    reg.profile(True)
+    reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=a.map2048_steps)
+    torch.cuda.synchronize()
     prof = reg.profile_report()
     reg.profile(False)
     reg.close()
@@ -428,20 +430,22 @@ def main():
     if (sm.dx, sm.dy) != (float(shift[0]), float(shift[1])) or (sm.n, sm.ny) != (2048, 2048):
         raise SystemExit(f"bench workload check failed: peak ({sm.dx}, {sm.dy}), grid {sm.n}x{sm.ny}")
 
-    # warmup, then exactly K maps bracketed by device synchronization, per-kernel HIP events on
-    # the engine's stream over the timed region
+    # warmup, then exactly K maps bracketed by device synchronization (the engine's streams as
+    # in use: no per-launch events); then the same K maps again with a HIP event around every
+    # launch (one stream) for the per-kernel times of the roofline
     reg.bench_shift_map(H, exact=True, calibrated=False, resolution=2048, iters=a.warmup)
-    torch.cuda.synchronize()
-    reg.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev_ms = reg.bench_shift_map(H, exact=True, calibrated=False, resolution=2048, iters=a.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = reg.profile_report()
-    reg.profile(False)
     wall = t1 - t0
     maps_s = a.steps / wall
+    reg.profile(True)
+    reg.bench_shift_map(H, exact=True, calibrated=False, resolution=2048, iters=a.steps)
+    torch.cuda.synchronize()
+    prof = reg.profile_report()
+    reg.profile(False)
 
     rect = (sm.cw, sm.ch) == tuple(sm.canvas)
     # the workload's pose is the identity: an integer translation onto the canvas

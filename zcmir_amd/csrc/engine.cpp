@@ -85,6 +85,9 @@ Engine::Engine(int device) : dev(device) {
   if (device < 0 || device >= n) throw std::runtime_error("no HIP device " + std::to_string(device));
   ZC_HIP(hipSetDevice(dev));
   ZC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  ZC_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+  ZC_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  ZC_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   hipDeviceProp_t p;
   ZC_HIP(hipGetDeviceProperties(&p, dev));
   device_name = std::string(p.name) + " (" + p.gcnArchName + ")";
@@ -94,16 +97,20 @@ Engine::Engine(int device) : dev(device) {
 Engine::~Engine() {
   (void)hipSetDevice(dev);
   if (st) (void)hipStreamSynchronize(st);
+  if (st2) (void)hipStreamSynchronize(st2);
   for (auto& s : sides) release_side(s);
   for (auto& s : stack) release_side(*s);
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
                   &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
                   &gn_sums, &sat_band, &sat, &rs_fwd, &rs_inv, &rs_out, &rs_tab, &pre_buf, &pre_hist, &pre_cdf,
-                  &pre_part, &pre_rgba})
+                  &pre_part, &pre_rgba, &map_out})
     b->release();
   sweep_gc.reset();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (st2) (void)hipStreamDestroy(st2);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -568,12 +575,17 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
   LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
   bool ok = true;
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st));
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st));
-  LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
+  // The summed-area tables depend only on the canvases: unless launches are profiled (one event
+  // chain on one stream), they are built on the second stream while the FFT stages run.
+  const bool fork = rect && !prof.on;
+  hipStream_t sst = fork ? st2 : st;
+  if (fork) {
+    ZC_HIP(hipEventRecord(ev_fork, st));
+    ZC_HIP(hipStreamWaitEvent(st2, ev_fork, 0));
+  }
   SatView sv{}, svb{};
   if (rect) {
-    LAUNCH("sat_build", ok &= launch_sat_build(plan, 0, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), st));
+    LAUNCH("sat_build", ok &= launch_sat_build(plan, 0, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), sst));
     sv.sat = sat.as<double>();
     sv.W = W;
     sv.H = Hh;
@@ -590,7 +602,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   }
   if (trans) {
     double* satb = sat.as<double>() + (int64_t)pi.n_sat * (W + 1) * (Hh + 1);
-    LAUNCH("sat_build", ok &= launch_sat_build(plan, 1, canvas_b.as<float>(), W, Hh, sat_band.as<double>(), satb, st));
+    LAUNCH("sat_build", ok &= launch_sat_build(plan, 1, canvas_b.as<float>(), W, Hh, sat_band.as<double>(), satb, sst));
     svb = sv;
     svb.sat = satb;
     // the moving image copied to canvas cells [tx, tx + w) x [ty, ty + h); its tables are read over
@@ -602,6 +614,10 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     svb.by1 = std::min(std::max(0, ty + A.h), Hh);
     svb.sign = -1;
   }
+  if (fork) ZC_HIP(hipEventRecord(ev_join, st2));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st));
+  LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   LAUNCH("cmul", launch_cmul(plan, spa, spb, work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
   LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
@@ -615,6 +631,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   ca.ridge = kRidge;
   const double gcell = (double)W / (double)cw;
   ca.area = std::max(1.0, pair_area(A, B) / (gcell * gcell));
+  if (fork) ZC_HIP(hipStreamWaitEvent(st, ev_join, 0));
   LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st, rect ? &sv : nullptr, trans ? &svb : nullptr));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));

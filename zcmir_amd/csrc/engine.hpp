@@ -236,6 +236,10 @@ class Engine {
 
   int dev;
   hipStream_t st = nullptr;
+  // a second stream (default priority) for work that runs beside the main sequence: the map's
+  // summed-area tables; ev_fork / ev_join order it against st
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Settings set;
   Side sides[2];
   Mat3 H = {1, 0, 0, 0, 1, 0, 0, 0, 1};
