@@ -82,3 +82,27 @@ def test_fine_cells_cover_the_coarse_rotation():
     assert prior.fine_cells([(0, 4)], margin=1) == [(70, 4), (71, 4), (0, 4), (1, 4), (2, 4)]
     assert prior.truth_cell(-180.0, 1.0) == (0, 4) and prior.truth_cell(172.6, 0.8) == (0, 0)
     assert prior.truth_cell(37.0, 1.12) == (14, 6)
+
+
+def test_fine_cells_follow_the_grid_by_angle_and_scale():
+    """The proposals map onto any sweep grid by value: at the registrar's default 90 rotations
+    (4° steps) class 0 (−180°) covers the rotations within ±7.5° (−184°, −180°, −176°), class 23
+    (165°) four rotations (160°..172°), never the 12°-per-class cells an index rule would give; a
+    5-scale grid over the same range maps a class to the scale at its value, a class between two
+    grid scales to the nearer; a partial rotation range keeps only its own rotations."""
+    from zcmir_amd import prior
+
+    grid = {**prior.TRAIN_GRID, "sw_nth": 90}
+    assert prior.fine_cells([(0, 4)], grid) == [(89, 4), (0, 4), (1, 4)]
+    assert prior.fine_cells([(23, 4)], grid) == [(85, 4), (86, 4), (87, 4), (88, 4)]
+    th, sg, wrap = prior.sweep_axes(grid)
+    assert wrap and len(th) == 90 and abs(th[85] - 160.0) < 1e-12 and abs(sg[4] - 1.0) < 1e-12
+    five = {**prior.TRAIN_GRID, "sw_nth": 72, "sw_ns": 5}
+    assert prior.fine_cells([(12, 2)], five) == [(35, 1), (36, 1), (37, 1)]     # 0°, scale 0.8^(1/2)
+    assert prior.fine_cells([(12, 8)], five) == [(35, 4), (36, 4), (37, 4)]     # scale 1.25
+    assert prior.fine_cells([(12, 3)], five) == [(35, 1), (36, 1), (37, 1)]     # between: the nearer (first)
+    part = {**prior.TRAIN_GRID, "sw_rot_range": True, "sw_rot_lo": -30.0, "sw_rot_hi": 30.0, "sw_nth": 13}
+    cells = prior.fine_cells([(12, 4), (10, 4)], part)   # 0° and −30°
+    ths = prior.sweep_axes(part)[0]
+    assert [ths[j] for j, _ in cells] == [-5.0, 0.0, 5.0, -30.0, -25.0]
+    assert cells == [(5, 4), (6, 4), (7, 4), (0, 4), (1, 4)]

@@ -193,17 +193,67 @@ def propose(net, moving, fixed, k=8):
     return out
 
 
-def fine_cells(cells, n_theta=72, margin=0):
-    """The (rotation, scale) cells of an n_theta-rotation sweep grid (same 9 scales) inside the
-    proposed 15° cells: each coarse rotation index covers the fine rotations within ±7.5°, and
-    `margin` more fine rotations on each side (a pose near a coarse cell's edge)."""
-    per = n_theta // N_TH
+# the sweep grid the network's classes are labelled on: 24 rotations over the circle, 9 scales
+# log-spaced over 0.8 – 1.25
+TRAIN_GRID = {"sw_rot_range": False, "sw_rot": 180.0, "sw_nth": N_TH, "sw_smin": 0.8, "sw_smax": 1.25, "sw_ns": N_SG}
+
+
+def sweep_axes(settings):
+    """The rotation (degrees) and scale axes of a sweep grid from sweep settings, with the clamps
+    and spacing of the engine's sweep (search.zig:89-116 sweepGrid): the full circle without
+    repeating ±180°, a partial range with both ends; scales log-spaced. Returns (thetas, sigmas,
+    wrap)."""
+    cl = lambda v, lo, hi: min(max(v, lo), hi)   # noqa: E731
+    rot = cl(float(settings["sw_rot"]), 0.0, 180.0)
+    if settings["sw_rot_range"]:
+        lo, hi = cl(float(settings["sw_rot_lo"]), -360.0, 360.0), cl(float(settings["sw_rot_hi"]), -360.0, 360.0)
+    else:
+        lo, hi = -rot, rot
+    if hi < lo:
+        hi += 360.0
+    hi = min(hi, lo + 360.0)
+    n = cl(int(settings["sw_nth"]), 1, 720)
+    span = abs(hi - lo)
+    wrap = span >= 360.0 - 1e-9
+    if wrap:
+        thetas = [-180.0 + 360.0 * i / n for i in range(n)]
+    elif n == 1:
+        thetas = [lo + span / 2]
+    else:
+        thetas = [lo + span * i / (n - 1) for i in range(n)]
+    smin, smax = cl(float(settings["sw_smin"]), 0.05, 1.0), cl(float(settings["sw_smax"]), 1.0, 20.0)
+    ns = cl(int(settings["sw_ns"]), 1, 129)
+    a, b = math.log(max(1e-3, min(smin, smax))), math.log(max(1e-3, max(smin, smax)))
+    sigmas = [math.exp((a + b) / 2 if ns == 1 else a + (b - a) * k / (ns - 1)) for k in range(ns)]
+    return thetas, sigmas, wrap
+
+
+def fine_cells(cells, settings=None, margin=0):
+    """The (rotation index, scale index) cells of a sweep grid (settings: the registrar's sweep
+    settings; None: 72 rotations × the training grid's 9 scales) inside the network's proposed
+    cells: for a class (it, is) every rotation within ±7.5° of −180° + 15°·it, widened by `margin`
+    of the grid's rotation steps on each side (a pose near a class's edge), and every scale within
+    half a class step (a factor 1.25^(1/8)) of 1.25^((is − 4)/4). Cells keep their first-seen order."""
+    if settings is None:
+        settings = {**TRAIN_GRID, "sw_nth": 72}
+    thetas, sigmas, wrap = sweep_axes(settings)
+    step = (360.0 / len(thetas)) if wrap else (abs(thetas[-1] - thetas[0]) / max(len(thetas) - 1, 1))
+    half_class = math.log(1.25) / 8
     out = []
     for it, is_ in cells:
-        for d in range(-(per // 2) - margin, per - per // 2 + margin):
-            c = ((per * it + d) % n_theta, is_)
-            if c not in out:
-                out.append(c)
+        centre = -180.0 + 15.0 * it
+        reach = 7.5 + margin * step + 1e-9
+        rot_idx = [j for j, t in enumerate(thetas) if abs((t - centre + 180.0) % 360.0 - 180.0) <= reach]
+        # rotations in order around the class centre, from its lower side
+        rot_idx.sort(key=lambda j: (thetas[j] - centre + 180.0) % 360.0 - 180.0)
+        ls = math.log(1.25) * (is_ - 4) / 4
+        sc_idx = [k for k, sg in enumerate(sigmas) if abs(math.log(sg) - ls) <= half_class + 1e-9]
+        if not sc_idx:   # no scale of the grid inside the class: its nearest
+            sc_idx = [min(range(len(sigmas)), key=lambda k: abs(math.log(sigmas[k]) - ls))]
+        for j in rot_idx:
+            for k in sc_idx:
+                if (j, k) not in out:
+                    out.append((j, k))
     return out
 
 
@@ -234,7 +284,7 @@ def calibrate(reg, net, moving, fixed, poses, thetas, sigmas, q=0.05):
     for i in range(len(poses)):
         reg.set_images(moving[i].cpu().numpy(), fixed[i].cpu().numpy(), preprocess=False)
         t, s_ = truth_cell(thetas[i], sigmas[i])
-        vals.append(check_score(reg, fine_cells([(t, s_)], int(reg.settings["sw_nth"]))))
+        vals.append(check_score(reg, fine_cells([(t, s_)], reg.settings)))
     return float(np.quantile(vals, q))
 
 
@@ -248,7 +298,7 @@ def learned_search(reg, net, moving, fixed, k=8, threshold=0.0, margin=1):
     Returns (Search, info dict: proposed cells, fine cells, check score, fell_back)."""
     H0 = reg.pose
     cells = propose(net, moving[None], fixed[None], k)[0]
-    fine = fine_cells(cells, int(reg.settings["sw_nth"]), margin)
+    fine = fine_cells(cells, reg.settings, margin)
     score = check_score(reg, fine)
     fell_back = score < 0.8 * threshold
     reg.pose = H0
