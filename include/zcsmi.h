@@ -96,6 +96,16 @@ void zc_swap(void* e);
 /* ── scores and maps ── */
 /* exports.zig:276: the 45 overlap sums of moving onto fixed at H */
 int32_t zc_moments(void* e, const double H[9], double out[45]);
+/* No counterpart in the reference: zc_moments for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops). feat_a / feat_b: device pointers to 4 planes of
+ * wa x ha / wb x hb float32 features (zc_features' layout); H on the host; out: a device pointer
+ * to 45 doubles, written when the work queued on `stream` (NULL: the default stream) has run. The
+ * partial sums live in a fixed 360 KiB device array of the library (1024 rows of 45 doubles, one
+ * per device, kept until the process ends): no allocation happens, and calls that overlap on one
+ * device race on that array. Returns 0, -1 for an invalid argument (null pointer, empty image,
+ * more than 2^31 pixels), -2 for a device or launch failure. */
+int32_t zc_moments_noalloc(const float* feat_a, uint32_t wa, uint32_t ha, const float* feat_b, uint32_t wb,
+                           uint32_t hb, const double H[9], double* out, void* stream);
 /* exports.zig:417: [mean, forward, inverse] (inverse NaN when not symmetric) */
 int32_t zc_score(void* e, const double H[9], double out[3]);
 /* exports.zig:424: score, overlap and tangent gradient at H (hess: also the Gauss-Newton

@@ -141,6 +141,7 @@ _SIGS = {
     "zc_sweep_map": ([P, f32p, SZ, u32p], I32),
     "zc_sweep_tune": ([U32], None),
     "zc_map_path": ([U32], None),
+    "zc_moments_noalloc": ([P, U32, U32, P, U32, U32, f64p, P, P], I32),
     "zc_stack_set": ([P, U32, P, u32p, P, u32p], I32),
     "zc_stack_register": ([P, f64p, U32, U32, f64p, f64p, f64p], I32),
     "zc_set_image_device": ([P, U32, P, U32, U32], I32),
