@@ -153,3 +153,17 @@ def test_sweep_window_reproduces_ramps_at_cell_centres(k, cell):
                 assert abs(got[1 + p, cy, cx] - (a + b * sx + c * sy)) < 1e-9, (k, cell, cx, cy, p)
             checked += 1
     assert checked > 200
+
+
+def test_cpu_port_sweep_matches_oracle():
+    """The float32 CPU port of the sweep (the bench's CPU baseline) computes the oracle's sweep:
+    every candidate's best peak within float32 rounding."""
+    from oracle import cpu_port
+
+    mv, fx, _ = synth.rotscale_pair((96, 96), theta_deg=20.0, scale=1.05, seed=4)
+    pm, pf = O.prepare(mv), O.prepare(fx)
+    gv = O.sweep_grid(-40, 40, 3, 0.9, 1.1, 2)
+    r = O.sweep(pm, pf, np.eye(3), gv, grid=32)
+    done, best = cpu_port.sweep_f32(pm, pf, np.eye(3), gv, grid=32, workers=2)
+    assert done == 6
+    assert np.abs(best - r["best"]).max() <= 2e-4 * r["best"].max()

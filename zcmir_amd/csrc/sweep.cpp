@@ -257,7 +257,9 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
       bytes("cmul", b * (2 * NCd + Qd) * nnd * 8);
       bytes("colfft_inv", b * 2 * Qd * nnd * 8);
       bytes("rowifft_extract", b * (Qd * nnd * 8 + Cd * nnd * 4));
-      bytes("combine_sweep", b * 2 * nnd * 4);   // lower bound: n and the score of every lag
+      // every correlation plane at every lag (lags under the floor read only n, but their cache
+      // lines hold lags above it) and the score written
+      bytes("combine_sweep", b * (Cd + 1) * nnd * 4);
       bytes("peaks", b * nnd * 4);
       LAUNCH("pack_sweep", ok &= launch_pack_sweep(plan, A.il.as<float4>(), dposes + c0, b, um, (int)nx,
                                                     spec_a.as<float2>(), tx, st));
