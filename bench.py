@@ -20,6 +20,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32; single-issue FMA (no packed FP32 math) tops out at half
+# Algorithmic FP32-equivalent operations per destination pixel and pose (counted from the kernels'
+# arithmetic, adds and multiplies one each, an FMA two): pose moments = projection 15 + taper 8 +
+# 4 bilinear samples 36 + the 45 weighted monomials 89; tangent pass with the Gauss-Newton matrix =
+# projection 15 + taper 20 + samples and their gradients 68 + residual 100 + (cx, cy) 20 +
+# 8 tangent derivatives 144 + gradient 32 + the 36-entry matrix 288.
+POSE_MOMENT_FLOPS = 148
+TANGENT_FLOPS = 687
 
 
 def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
@@ -202,9 +210,31 @@ def bench_stack(a, np, torch, zcmir_amd, synth):
         out["init_" + init] = {"register_s": round(t_reg, 3), "slices_per_s": round(len(sl) / t_reg, 2),
                                "iterations": int(it), "within_1px": round(float(np.mean(errs < 1.0)), 3),
                                "median_corner_err_px": round(float(np.median(errs)), 3)}
+    # a profiled registration (HIP events around every launch), apart from the timed ones
+    reg.profile(True)
+    reg.register_stack(iters=30, init="sweep")
+    prof, pbytes, pwork = reg.profile_report(), reg.profile_bytes(), reg.profile_work()
+    reg.profile(False)
     reg.close()
+    flops = {"pose_moments": POSE_MOMENT_FLOPS, "tangent_grad": TANGENT_FLOPS}
+    kern = {}
+    for name, (cnt, tot, avg) in sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]:
+        row = {"launches": cnt, "total_ms": round(tot, 3), "avg_ms": round(avg, 4)}
+        if name in flops and pwork.get(name):
+            row["pixel_poses"] = pwork[name]
+            row["tflops"] = round(pwork[name] * flops[name] / (tot * 1e-3) / 1e12, 2)
+            row["feature_bytes"] = pbytes.get(name)
+            row["gbs_features"] = round(pbytes.get(name, 0) / (tot * 1e-3) / 1e9, 1)
+        kern[name] = row
+    out["kernels_profiled_registration"] = kern
+    d = kern.get("pose_moments")
+    if d and "tflops" in d:
+        out["roofline"] = {"bound": "valu", "kernel": "pose_moments", "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(d["tflops"] / FP32_PEAK_TFLOPS, 4),
+                           "flops_per_pixel_pose": POSE_MOMENT_FLOPS, "traffic": None}
     out["note"] = ("init_map: each start corrected by its shift map's peak (translation only; rotations up to "
-                   "10 deg are left to Gauss-Newton); init_sweep: by a +-12 deg, 0.88-1.12 sweep's best peak")
+                   "10 deg are left to Gauss-Newton); init_sweep: by a +-12 deg, 0.88-1.12 sweep's best peak. "
+                   "Kernel rows come from one more registration (sweep starts) with HIP events around every launch.")
     return out
 
 

@@ -103,7 +103,7 @@ int32_t zc_moments(void* e, const double H[9], double out[45]);
  * partial sums live in a fixed 360 KiB device array of the library (1024 rows of 45 doubles, one
  * per device, kept until the process ends): no allocation happens, and calls that overlap on one
  * device race on that array. Returns 0, -1 for an invalid argument (null pointer, empty image,
- * more than 2^31 pixels), -2 for a device or launch failure. */
+ * more than 2^30 pixels), -2 for a device or launch failure. */
 int32_t zc_moments_noalloc(const float* feat_a, uint32_t wa, uint32_t ha, const float* feat_b, uint32_t wb,
                            uint32_t hb, const double H[9], double* out, void* stream);
 /* exports.zig:417: [mean, forward, inverse] (inverse NaN when not symmetric) */

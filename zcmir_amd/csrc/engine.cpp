@@ -63,8 +63,10 @@ std::string Prof::report() const {
   for (const auto& kv : acc_) {
     char line[256];
     const auto b = bytes_.find(kv.first);
-    std::snprintf(line, sizeof line, "%s %ld %.6f %.6f %.0f\n", kv.first.c_str(), kv.second.first, kv.second.second,
-                  kv.second.second / std::max(1L, kv.second.first), b == bytes_.end() ? 0.0 : b->second);
+    const auto u = work_.find(kv.first);
+    std::snprintf(line, sizeof line, "%s %ld %.6f %.6f %.0f %.0f\n", kv.first.c_str(), kv.second.first,
+                  kv.second.second, kv.second.second / std::max(1L, kv.second.first),
+                  b == bytes_.end() ? 0.0 : b->second, u == work_.end() ? 0.0 : u->second);
     o << line;
   }
   return o.str();
@@ -317,7 +319,7 @@ void Engine::bake(Side& sd) {
 }
 
 void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) {
-  if (s < 0 || s > 1 || w < 8 || h < 8 || !data) throw std::runtime_error("BadImage");
+  if (s < 0 || s > 1 || w < 8 || h < 8 || !data || (int64_t)w * h > kMaxImagePixels) throw std::runtime_error("BadImage");
   Side& sd = sides[s];
   const size_t n = (size_t)w * h;
   sd.w = w;
@@ -341,7 +343,8 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
 // exports.zig:60-63 / engine.zig:422-427: RGBA8 → gray (0.299 R + 0.587 G + 0.114 B), then the
 // work image always preprocessed per the settings.
 void Engine::set_image_rgba(int s, const uint8_t* rgba, int w, int h) {
-  if (s < 0 || s > 1 || w < 8 || h < 8 || !rgba) throw std::runtime_error("BadImage");
+  if (s < 0 || s > 1 || w < 8 || h < 8 || !rgba || (int64_t)w * h > kMaxImagePixels)
+    throw std::runtime_error("BadImage");
   const size_t n = (size_t)w * h;
   pre_rgba.ensure(n * 4);
   ZC_HIP(hipMemcpyAsync(pre_rgba.p, rgba, n * 4, hipMemcpyHostToDevice, st));

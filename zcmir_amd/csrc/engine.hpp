@@ -113,9 +113,12 @@ class Prof {
   void reset() {
     acc_.clear();
     bytes_.clear();
+    work_.clear();
   }
   // algorithmic HBM bytes of a launch (minimal traffic of its role), summed per kernel name
   void add_bytes(const char* name, double b) { bytes_[name] += b; }
+  // work units of a launch (pose kernels: destination pixels × poses), summed per kernel name
+  void add_work(const char* name, double u) { work_[name] += u; }
   ~Prof();
 
  private:
@@ -129,12 +132,15 @@ class Prof {
   hipEvent_t last_ = nullptr;   // the end event of the previous launch, when nothing came between
   std::map<std::string, std::pair<long, double>> acc_;
   std::map<std::string, double> bytes_;
+  std::map<std::string, double> work_;
   hipEvent_t get();
 };
 
 inline constexpr double kRidge = 1e-4;   // smi.zig:18
 inline constexpr int kMinN = 12;          // smi.zig:17
 inline constexpr int kMaxFft = 4096;      // longest FFT line the kernels implement
+// Largest image (pixels): the pose kernels address a feature plane with 32-bit byte offsets.
+inline constexpr int64_t kMaxImagePixels = int64_t(1) << 30;
 
 struct Side {
   int w = 0, h = 0;
@@ -143,8 +149,10 @@ struct Side {
   DBuf raw;          // the image as given (f32 gray), kept to bake again when settings change
   std::string baked; // the settings the work image and features were made with
   DBuf gray, rank, feat, mag, stack;
-  DBuf il;   // features interleaved per pixel (float4) for the sweep's gathers
-  bool il_ok = false;
+  // features interleaved per pixel (float4) for the sweep's and the pose kernels' gathers, made
+  // on first use (a cache of feat: const users may fill it)
+  mutable DBuf il;
+  mutable bool il_ok = false;
   float mean = 0, spread = 1;
   uint32_t hist[257] = {};
   uint32_t cdf[258] = {};
@@ -349,6 +357,9 @@ class Engine {
   void bytes(const char* name, double b) {
     if (prof.on) prof.add_bytes(name, b);
   }
+  void count_work(const char* name, double u) {
+    if (prof.on) prof.add_work(name, u);
+  }
   void post(const char* name) {
     ++counts[0];
     if (prof.on) prof.end(name, st);
@@ -370,7 +381,7 @@ class Engine {
   int autotune_cols(int L);
   DBuf tune_buf;
   void ensure_acf(Side& sd);
-  void ensure_il(Side& sd);
+  void ensure_il(const Side& sd);
 
  private:
   void prepare(Side& sd, bool acf);

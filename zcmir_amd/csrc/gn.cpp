@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <set>
 
 #include "engine.hpp"
 #include "lie.hpp"
@@ -170,20 +171,31 @@ void Engine::moments_jobs(const std::vector<DirJob>& d, std::vector<std::array<d
   if (!nj) return;
   std::vector<PoseJob> jobs(nj);
   int64_t px = 1;
+  // work: destination pixels over all poses; algorithmic bytes: each side's 4 feature planes once
+  double pixels = 0, feat_bytes = 0;
+  std::set<const Side*> seen;
   for (size_t i = 0; i < nj; ++i) {
     const Side& a = *d[i].a;
     const Side& b = *d[i].b;
     const Mat3 Hi = inv3(d[i].H);
-    jobs[i] = {a.feat.as<float>(), b.feat.as<float>(), a.w, a.h, b.w, b.h, {}};
+    ensure_il(*d[i].a);
+    ensure_il(*d[i].b);
+    jobs[i] = {a.il.as<float4>(), b.il.as<float4>(), a.w, a.h, b.w, b.h, {}};
     for (int k = 0; k < 9; ++k) jobs[i].Hi[k] = Hi[k];
     px = std::max<int64_t>(px, (int64_t)b.w * b.h);
+    pixels += (double)b.w * b.h;
+    seen.insert(&a);
+    seen.insert(&b);
   }
+  for (const Side* s : seen) feat_bytes += 16.0 * s->w * s->h;
   const int blocks = jobs_blocks(px);
   gn_jobs.ensure(nj * sizeof(PoseJob));
   gn_part.ensure(nj * blocks * 45 * sizeof(double));
   gn_sums.ensure(nj * 45 * sizeof(double));
   ZC_HIP(hipMemcpyAsync(gn_jobs.p, jobs.data(), nj * sizeof(PoseJob), hipMemcpyHostToDevice, st));
   prof.break_chain();
+  bytes("pose_moments", feat_bytes);
+  count_work("pose_moments", pixels);
   LAUNCH("pose_moments", launch_pose_moments_jobs(gn_jobs.as<PoseJob>(), (int)nj, blocks, gn_part.as<double>(),
                                                   gn_sums.as<double>(), st));
   ZC_HIP(hipMemcpyAsync(out.data(), gn_sums.p, nj * 45 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -199,13 +211,17 @@ void Engine::tangent_jobs(const std::vector<DirJob>& d, const std::vector<Coefs>
   if (!nj) return;
   std::vector<TangentJob> jobs(nj);
   int64_t px = 1;
+  double pixels = 0, feat_bytes = 0;
+  std::set<const Side*> seen;
   for (size_t i = 0; i < nj; ++i) {
     const Side& a = *d[i].a;
     const Side& b = *d[i].b;
     TangentJob& j = jobs[i];
     std::memset(&j, 0, sizeof j);
-    j.fa = a.feat.as<float>();
-    j.fb = b.feat.as<float>();
+    ensure_il(*d[i].a);
+    ensure_il(*d[i].b);
+    j.fa = a.il.as<float4>();
+    j.fb = b.il.as<float4>();
     j.wa = a.w;
     j.ha = a.h;
     j.wb = b.w;
@@ -230,13 +246,19 @@ void Engine::tangent_jobs(const std::vector<DirJob>& d, const std::vector<Coefs>
     }
     for (int q = 0; q < 45; ++q) j.g[q] = (float)c.g[q];
     px = std::max<int64_t>(px, (int64_t)b.w * b.h);
+    pixels += (double)b.w * b.h;
+    seen.insert(&a);
+    seen.insert(&b);
   }
+  for (const Side* s : seen) feat_bytes += 16.0 * s->w * s->h;
   const int blocks = jobs_blocks(px);
   gn_jobs.ensure(nj * sizeof(TangentJob));
   gn_part.ensure(nj * blocks * 44 * sizeof(double));
   gn_sums.ensure(nj * 44 * sizeof(double));
   ZC_HIP(hipMemcpyAsync(gn_jobs.p, jobs.data(), nj * sizeof(TangentJob), hipMemcpyHostToDevice, st));
   prof.break_chain();
+  bytes("tangent_grad", feat_bytes);
+  count_work("tangent_grad", pixels);
   LAUNCH("tangent_grad", launch_tangent_jobs(gn_jobs.as<TangentJob>(), (int)nj, blocks, gn_part.as<double>(),
                                              gn_sums.as<double>(), st));
   ZC_HIP(hipMemcpyAsync(out.data(), gn_sums.p, nj * 44 * sizeof(double), hipMemcpyDeviceToHost, st));

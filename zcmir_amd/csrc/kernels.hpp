@@ -133,18 +133,19 @@ void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_
                          double* part, hipStream_t st);
 
 // ── batched pose sums and gradients (one pose per job) ──
-// moving features fa (wa × ha, 4 planes) sampled at Hi · X for every pixel X of fb (wb × hb)
+// moving features fa (wa × ha) sampled at Hi · X for every pixel X of fb (wb × hb); fa / fb: the
+// sides' interleaved features (one float4 per pixel)
 struct PoseJob {
-  const float* fa;
-  const float* fb;
+  const float4* fa;
+  const float4* fb;
   int wa, ha, wb, hb;
   double Hi[9];
 };
 // the tangent pass of one pose: Hi, M_k = −Hi · G_k (pixel generators of fb's image, zero past
 // the group's), the score's coefficients (W, M̂, Q row-major, the means, g = ∂S/∂sums)
 struct TangentJob {
-  const float* fa;
-  const float* fb;
+  const float4* fa;
+  const float4* fb;
   int wa, ha, wb, hb, hess, _pad;
   double Hi[9];
   float Mk[72];

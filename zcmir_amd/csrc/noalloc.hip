@@ -52,7 +52,7 @@ extern "C" {
 int32_t zc_moments_noalloc(const float* feat_a, uint32_t wa, uint32_t ha, const float* feat_b, uint32_t wb,
                            uint32_t hb, const double* H, double* out, void* stream) {
   if (!feat_a || !feat_b || !H || !out || wa == 0 || ha == 0 || wb == 0 || hb == 0) return -1;
-  if ((uint64_t)wa * ha > (1ull << 31) || (uint64_t)wb * hb > (1ull << 31)) return -1;
+  if ((uint64_t)wa * ha > (1ull << 30) || (uint64_t)wb * hb > (1ull << 30)) return -1;   // 32-bit byte offsets
   const int nb = zc::moments_blocks((int)wb, (int)hb);
   if (nb < 1 || nb > kMaxParts) return -1;
   int dev = 0;

@@ -18,7 +18,8 @@ void Engine::stack_set(uint32_t n, const float* const* mv, const uint32_t* mwh, 
     for (int k = 0; k < 2; ++k) {
       const float* data = k == 0 ? mv[i] : fx[i];
       const int w = (int)(k == 0 ? mwh[2 * i] : fwh[2 * i]), h = (int)(k == 0 ? mwh[2 * i + 1] : fwh[2 * i + 1]);
-      if (w < 8 || h < 8 || !data) throw std::runtime_error("BadImage: slice " + std::to_string(i));
+      if (w < 8 || h < 8 || !data || (int64_t)w * h > kMaxImagePixels)
+        throw std::runtime_error("BadImage: slice " + std::to_string(i));
       auto sd = std::make_unique<Side>();
       const size_t px = (size_t)w * h;
       sd->w = w;

@@ -17,7 +17,7 @@
 
 namespace zc {
 
-void Engine::ensure_il(Side& sd) {
+void Engine::ensure_il(const Side& sd) {
   if (sd.il_ok) return;
   const int64_t n = (int64_t)sd.w * sd.h;
   sd.il.ensure((size_t)n * 16);

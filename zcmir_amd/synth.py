@@ -10,7 +10,8 @@ them. Workloads:
   k / 255 (8-bit intensities).
 * ``rotscale_pair``: moving = fixed rotated by theta and scaled by s about the image centre
   (bilinear), then remapped by x^0.5.
-* ``stack_slices``: uneven-size slices, each with a random homography and intensity remap.
+* ``stack_slices``: uneven-size slices, each with a random homography and intensity remap, the
+  moving frame cut from the same texture as the fixed one (no empty border).
 * ``drift_stack``: square slices whose texture and pose drift slowly from slice to slice, the
   fixed side another modality (1 - x^0.6 plus noise) - the design of the reference's stack test
   (tests/test_api.py:186-202), re-created here.
@@ -127,7 +128,13 @@ def random_homography(rng: np.random.Generator, w: int, h: int, rot_deg=10.0, sc
 
 
 def stack_slices(n: int = 64, side=(512, 1536), max_aspect: float = 1.5, seed: int = 3):
-    """[(moving, fixed, H_true, remap name)] uneven-size slices (see module docstring)."""
+    """[(moving, fixed, H_true, remap name)] uneven-size slices (see module docstring).
+
+    Both frames are cut from one texture that extends 0.35 · max(h, w) + 8 px past the fixed
+    frame on every side, so every moving pixel holds real content: moving(X) = remap(tex(H_true X))
+    with the fixed frame at the texture's offset. (Filling the part of the moving frame that falls
+    outside the fixed image with zeros instead leaves an artificial edge there whose gradient and
+    Laplacian features pull the SMI optimum a median 0.8 px off H_true on these slices.)"""
     rng = np.random.default_rng(seed)
     out = []
     names = list(REMAPS)
@@ -136,9 +143,12 @@ def stack_slices(n: int = 64, side=(512, 1536), max_aspect: float = 1.5, seed: i
         asp = rng.uniform(1.0, max_aspect)
         b = int(max(side[0] // 2, min(side[1], round(a / asp))))
         h, w = (a, b) if rng.uniform() < 0.5 else (b, a)
-        fixed = _texture(h, w, rng)
+        m = int(0.35 * max(h, w)) + 8
+        tex = _texture(h + 2 * m, w + 2 * m, rng)
+        fixed = tex[m:m + h, m:m + w]
         Ht = random_homography(rng, w, h)         # moving → fixed
-        mv = warp(fixed, np.linalg.inv(Ht), (h, w))
+        to_tex = np.array([[1.0, 0.0, m], [0.0, 1.0, m], [0.0, 0.0, 1.0]]) @ Ht
+        mv = warp(tex, np.linalg.inv(to_tex), (h, w))
         name = names[int(rng.integers(len(names)))]
         moving = np.clip(REMAPS[name](np.clip(mv, 0.0, 1.0)), 0.0, 1.0)
         out.append((moving.astype(np.float32), fixed.astype(np.float32), Ht, name))
