@@ -36,6 +36,9 @@ def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
     N = nx * ny
     PA, PB, Q, C, T = plan["n_cplx_a"], plan["n_cplx_b"], plan["n_out"], plan["n_corr"], plan["n_sat"]
     TB = plan.get("n_bsat", 0)
+    # a translation plan reads both images' own tables (made once per image, S cells each); a
+    # rectangle plan builds canvas A's tables per map
+    tab_px = S if TB else canvas_px
     out = {
         "warp_stack": 20 * S + 20 * canvas_px,
         "paste_stack": 20 * S + 20 * canvas_px,
@@ -44,11 +47,11 @@ def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
         "cmul": (PA + PB) * N * 8 + Q * N * 8,
         "colfft_inv": 2 * Q * N * 8,
         "rowifft_extract": Q * N * 8 + C * N * 4,
-        "combine": C * N * 4 + N * 4 + (T + TB) * canvas_px * 8,
+        "combine": C * N * 4 + N * 4 + (T + TB) * tab_px * 8,
     }
-    if T:
-        # band sums and rows each read the canvas stack; the tables are written once (per side)
-        out["sat_build"] = 2 * 20 * canvas_px * (2 if TB else 1) + (T + TB) * canvas_px * 8
+    if T and not TB:
+        # band sums and rows each read the canvas stack; the tables are written once
+        out["sat_build"] = 2 * 20 * canvas_px + T * canvas_px * 8
     return out
 
 
@@ -508,10 +511,11 @@ def main():
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": round(dk["avg_ms"], 5)}
     total_alg = sum(kb[k] for k in kb if k in kernels)
     # whole-pipeline minimum: features in, spectra written and read back once, inverse spectra,
-    # the map, and the tables written and read once
+    # the map, and the tables read once (a rectangle plan also writes canvas A's once)
     model_bytes = 40 * size[0] * size[1] + 32 * (plan["n_cplx_a"] + plan["n_cplx_b"]) * sm.n * sm.ny \
         + 16 * plan["n_out"] * sm.n * sm.ny + 4 * sm.n * sm.ny \
-        + 16 * (plan["n_sat"] + plan.get("n_bsat", 0)) * sm.canvas[0] * sm.canvas[1]
+        + (8 * (plan["n_sat"] + plan["n_bsat"]) * size[0] * size[1] if plan.get("n_bsat")
+           else 16 * plan["n_sat"] * sm.canvas[0] * sm.canvas[1])
     survey_bytes = survey_model_bytes(size[0] * size[1], sm.n * sm.ny, sm.n * sm.ny)
 
     cpu = None

@@ -117,9 +117,22 @@ Engine::~Engine() {
 }
 
 void Engine::release_side(Side& s) {
-  for (DBuf* b : {&s.raw, &s.gray, &s.rank, &s.feat, &s.mag, &s.stack, &s.il}) b->release();
+  for (DBuf* b : {&s.raw, &s.gray, &s.rank, &s.feat, &s.mag, &s.stack, &s.il, &s.tab}) b->release();
   s.has = false;
   s.il_ok = false;
+  s.tab_ok = false;
+}
+
+// The image's own summed-area tables of its 15 stack planes (translation plans' rectangle sums).
+void Engine::ensure_tables(const Side& sd) {
+  if (sd.tab_ok) return;
+  sd.tab.ensure((size_t)sat_pairs(15) * (sd.w + 1) * (sd.h + 1) * 16);
+  sat_band.ensure((size_t)15 * sat_bands(sd.h) * sd.w * 8);
+  bool ok = true;
+  LAUNCH("image_tables", ok = launch_sat_build(0, 2, sd.stack.as<float>(), sd.w, sd.h, sat_band.as<double>(),
+                                               sd.tab.as<double2>(), st));
+  if (!ok) throw std::runtime_error("UnsupportedImageSize: image tables");
+  sd.tab_ok = true;
 }
 
 void Engine::GraphCache::reset() {
@@ -314,6 +327,7 @@ void Engine::bake(Side& sd) {
     prof.break_chain();
   }
   sd.il_ok = false;
+  sd.tab_ok = false;
   prepare(sd, true);
   sd.baked = bake_key(sd);
 }
@@ -326,6 +340,7 @@ void Engine::set_image(int s, const float* data, int w, int h, bool preprocess) 
   sd.h = h;
   sd.has = false;
   sd.il_ok = false;
+  sd.tab_ok = false;
   sd.preprocess = preprocess;
   sd.raw.ensure(n * 4);
   sd.gray.ensure(n * 4);
@@ -356,6 +371,7 @@ void Engine::set_image_rgba(int s, const uint8_t* rgba, int w, int h) {
   sd.h = h;
   sd.has = false;
   sd.il_ok = false;
+  sd.tab_ok = false;
   sd.preprocess = true;
   sd.gray.ensure(n * 4);
   sd.rank.ensure(n * 4);
@@ -561,9 +577,14 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   spec_a.ensure((size_t)N * 8 * (pi.n_cplx + pi.b_n_cplx));
   float2* const spa = spec_a.as<float2>();
   float2* const spb = spa + N * pi.n_cplx;
-  if (rect) {
-    sat_band.ensure((size_t)std::max(pi.n_sat, pi.n_bsat) * sat_bands(Hh) * W * 8);
-    sat.ensure((size_t)(pi.n_sat + pi.n_bsat) * (W + 1) * (Hh + 1) * 8);
+  if (rect && !trans) {
+    sat_band.ensure((size_t)pi.n_sat * sat_bands(Hh) * W * 8);
+    sat.ensure((size_t)sat_pairs(pi.n_sat) * (W + 1) * (Hh + 1) * 16);
+  }
+  // translation plans read each image's own tables (made once per image)
+  if (trans) {
+    ensure_tables(A);
+    ensure_tables(B);
   }
   work.ensure((size_t)N * 8 * pi.n_out);
   corr.ensure((size_t)N * 4 * pi.n_corr);
@@ -578,44 +599,40 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
   LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
   bool ok = true;
-  // The summed-area tables depend only on the canvases: unless launches are profiled (one event
+  // A rectangle plan's tables depend only on canvas A: unless launches are profiled (one event
   // chain on one stream), they are built on the second stream while the FFT stages run.
-  const bool fork = rect && !prof.on;
+  const bool fork = rect && !trans && !prof.on;
   hipStream_t sst = fork ? st2 : st;
   if (fork) {
     ZC_HIP(hipEventRecord(ev_fork, st));
     ZC_HIP(hipStreamWaitEvent(st2, ev_fork, 0));
   }
   SatView sv{}, svb{};
-  if (rect) {
-    LAUNCH("sat_build", ok &= launch_sat_build(plan, 0, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double>(), sst));
-    sv.sat = sat.as<double>();
-    sv.W = W;
-    sv.H = Hh;
-    sv.nx = nx;
-    sv.ny = ny;
-    sv.lx = 0;
-    while ((1 << sv.lx) < nx) ++sv.lx;
-    // the fixed image pasted at (-ox, -oy) (paste_stack)
-    sv.bx0 = std::max(0, -m.fr.ox);
-    sv.bx1 = std::min(W, -m.fr.ox + B.w);
-    sv.by0 = std::max(0, -m.fr.oy);
-    sv.by1 = std::min(Hh, -m.fr.oy + B.h);
-    sv.sign = 1;
+  int lx = 0;
+  while ((1 << lx) < nx) ++lx;
+  // the fixed image pasted at (-ox, -oy) (paste_stack): side A's sums run over this rectangle
+  const int fx0 = std::max(0, -m.fr.ox), fx1 = std::min(W, -m.fr.ox + B.w);
+  const int fy0 = std::max(0, -m.fr.oy), fy1 = std::min(Hh, -m.fr.oy + B.h);
+  if (rect && !trans) {
+    LAUNCH("sat_build", ok &= launch_sat_build(plan, 0, canvas_a.as<float>(), W, Hh, sat_band.as<double>(), sat.as<double2>(), sst));
+    sv = {sat.as<double2>(), nx, ny, lx, fx0, fx1, fy0, fy1, 1, 0, W, 0, Hh, W + 1, (uint32_t)((W + 1) * (Hh + 1))};
   }
   if (trans) {
-    double* satb = sat.as<double>() + (int64_t)pi.n_sat * (W + 1) * (Hh + 1);
-    LAUNCH("sat_build", ok &= launch_sat_build(plan, 1, canvas_b.as<float>(), W, Hh, sat_band.as<double>(), satb, sst));
-    svb = sv;
-    svb.sat = satb;
-    // the moving image copied to canvas cells [tx, tx + w) x [ty, ty + h); its tables are read over
-    // that rectangle moved by minus the lag
+    // Side A's sums: the fixed rectangle moved by the lag, over the moving image's own tables (the
+    // image is copied to canvas cells [tx, tx + w) x [ty, ty + h), so table cell = canvas cell −
+    // (tx, ty)), clipped to the image and to the canvas.
     const int tx = (int)Hc[2], ty = (int)Hc[5];
-    svb.bx0 = std::min(std::max(0, tx), W);
-    svb.bx1 = std::min(std::max(0, tx + A.w), W);
-    svb.by0 = std::min(std::max(0, ty), Hh);
-    svb.by1 = std::min(std::max(0, ty + A.h), Hh);
-    svb.sign = -1;
+    sv = {A.tab.as<double2>(), nx, ny, lx, fx0 - tx, fx1 - tx, fy0 - ty, fy1 - ty, 1,
+          std::max(0, -tx), std::min(A.w, W - tx), std::max(0, -ty), std::min(A.h, Hh - ty), A.w + 1,
+          (uint32_t)((A.w + 1) * (A.h + 1))};
+    // Side B's sums: the moving rectangle (clipped to the canvas) moved against the lag, over the
+    // fixed image's own tables (the image at (-ox, -oy)).
+    const int mx0 = std::min(std::max(0, tx), W), mx1 = std::min(std::max(0, tx + A.w), W);
+    const int my0 = std::min(std::max(0, ty), Hh), my1 = std::min(std::max(0, ty + A.h), Hh);
+    const int bx = -m.fr.ox, by = -m.fr.oy;
+    svb = {B.tab.as<double2>(), nx, ny, lx, mx0 - bx, mx1 - bx, my0 - by, my1 - by, -1,
+           std::max(0, -bx), std::min(B.w, W - bx), std::max(0, -by), std::min(B.h, Hh - by), B.w + 1,
+           (uint32_t)((B.w + 1) * (B.h + 1))};
   }
   if (fork) ZC_HIP(hipEventRecord(ev_join, st2));
   LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st));

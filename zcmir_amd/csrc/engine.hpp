@@ -153,6 +153,10 @@ struct Side {
   // on first use (a cache of feat: const users may fill it)
   mutable DBuf il;
   mutable bool il_ok = false;
+  // the image's own summed-area tables of its 15 stack planes (pairs), made on first use by a
+  // translation-plan map
+  mutable DBuf tab;
+  mutable bool tab_ok = false;
   float mean = 0, spread = 1;
   uint32_t hist[257] = {};
   uint32_t cdf[258] = {};
@@ -382,6 +386,7 @@ class Engine {
   DBuf tune_buf;
   void ensure_acf(Side& sd);
   void ensure_il(const Side& sd);
+  void ensure_tables(const Side& sd);
 
  private:
   void prepare(Side& sd, bool acf);

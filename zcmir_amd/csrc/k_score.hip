@@ -46,10 +46,12 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ corr,
 // When the fixed side's weight plane is the indicator of the rectangle R (the fixed image pasted
 // at full resolution), correlation (a, 0) at lag t is the sum of plane a over R + t clipped to the
 // canvas: four reads of plane a's summed-area table S(y, x) = sum over rows < y, columns < x
-// ((H+1) x (W+1) doubles per plane). The tables are built in three passes over bands of kSatBand
-// rows: every table's column sums per band (k_sat_band: one thread per column, all tables from
-// one read of the canvas), their exclusive prefix over bands (k_sat_carry), then every
-// (band, table) by a workgroup that scans its rows with a running column carry (k_sat_rows).
+// ((H+1) x (W+1) values per plane). Tables 2p and 2p+1 are stored interleaved as one double2 per
+// cell (table pair p; a last odd table pairs with zeros), so one 16-byte load at a corner serves
+// both. The tables are built in three passes over bands of kSatBand rows: every table's column
+// sums per band (k_sat_band: one thread per column, all tables from one read of the canvas), their
+// exclusive prefix over bands (k_sat_carry), then every (band, four tables) by a workgroup that
+// scans its rows with a running column carry (k_sat_rows).
 constexpr int kSatBand = 8;   // rows per band
 
 // The real planes of a canvas stack sample in FP64: w, w f_i, and w f_i f_j = s_i s_j / s_0
@@ -160,12 +162,12 @@ __device__ __forceinline__ void block_excl_scan(double (&v)[G], double* lds) {
   __syncthreads();   // lds is reused by the next scan
 }
 
-// blockIdx.x band, blockIdx.y a group of G tables (one read of the canvas for all of them);
-// thread t owns columns [t·E, t·E + E) for the scans; each finished row goes out through LDS so
-// that lane i stores column x + i (whole cache lines per store instruction).
+// blockIdx.x band, blockIdx.y a group of G tables (G / 2 table pairs; one read of the canvas for
+// all of them); thread t owns columns [t·E, t·E + E) for the scans; each finished row goes out
+// through LDS so that lane i stores column x + i of a pair (whole cache lines per store).
 template <int SID, int NT, int E, int G, int K0>
 __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, int W, int H,
-                                              const double* __restrict__ band, double* sat, double* lds,
+                                              const double* __restrict__ band, double2* sat, double* lds,
                                               double (*stage)[NT * E]) {
   constexpr SatSet P = sat_set(SID);
   constexpr int n_tab = P.n, k0 = K0;
@@ -173,8 +175,8 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
   const int xs = threadIdx.x * E;
   const int64_t n = (int64_t)W * H, SW = W + 1, SP = SW * (H + 1);
   constexpr int pl[G] = {P.plane[k0 < n_tab ? k0 : 0], P.plane[k0 + 1 < n_tab ? k0 + 1 : 0],
-                         P.plane[k0 + 2 < n_tab ? k0 + 2 : 0]};
-  static_assert(G == 3, "three tables per workgroup");
+                         P.plane[k0 + 2 < n_tab ? k0 + 2 : 0], P.plane[k0 + 3 < n_tab ? k0 + 3 : 0]};
+  static_assert(G == 4 && K0 % 2 == 0, "two table pairs per workgroup");
   // column carry cc[g][e] = S_g(y, xs + e + 1): the row prefix of the earlier bands' column sums
   double cc[G][E];
   {
@@ -197,9 +199,9 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
   }
   if (b == 0)
 #pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (k0 + g < n_tab)
-        for (int x = threadIdx.x; x <= W; x += NT) sat[(k0 + g) * SP + x] = 0.0;
+    for (int h = 0; h < G / 2; ++h)
+      if (k0 + 2 * h < n_tab)
+        for (int x = threadIdx.x; x <= W; x += NT) sat[(k0 / 2 + h) * SP + x] = make_double2(0.0, 0.0);
   const int y0 = b * kSatBand, y1 = min(H, y0 + kSatBand);
   // the canvas values of the thread's columns, one row ahead of the scans (E <= 4: the registers
   // of a second row fit)
@@ -244,11 +246,12 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
       }
     __syncthreads();
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      if (k0 + g >= n_tab) continue;
-      double* row = sat + (k0 + g) * SP + (int64_t)(y + 1) * SW;
-      for (int x = threadIdx.x; x < W; x += NT) row[x + 1] = stage[g][x];
-      if (threadIdx.x == 0) row[0] = 0.0;
+    for (int h = 0; h < G / 2; ++h) {
+      if (k0 + 2 * h >= n_tab) continue;
+      const bool hi = k0 + 2 * h + 1 < n_tab;   // else the pair's second table is padding
+      double2* row = sat + (k0 / 2 + h) * SP + (int64_t)(y + 1) * SW;
+      for (int x = threadIdx.x; x < W; x += NT) row[x + 1] = make_double2(stage[2 * h][x], hi ? stage[2 * h + 1][x] : 0.0);
+      if (threadIdx.x == 0) row[0] = make_double2(0.0, 0.0);
     }
   }
 #undef ZC_SAT_LOAD_ROW
@@ -258,39 +261,39 @@ __device__ __forceinline__ void sat_rows_body(const float* __restrict__ canvas, 
 // columns each cover rows of up to NT·E cells.
 template <int SID, int NT, int E, int G>
 __global__ __launch_bounds__(NT) void k_sat_rows(const float* __restrict__ canvas, int W, int H,
-                                                 const double* __restrict__ band, double* sat) {
+                                                 const double* __restrict__ band, double2* sat) {
   __shared__ double lds[NT / 64 * G];
   __shared__ double stage[G][NT * E];
-  static_assert(sat_set(SID).n <= 5 * G, "at most five groups");
+  static_assert(sat_set(SID).n <= 4 * G, "at most four groups");
   switch (blockIdx.y) {
     case 0: sat_rows_body<SID, NT, E, G, 0>(canvas, W, H, band, sat, lds, stage); break;
     case 1: sat_rows_body<SID, NT, E, G, G>(canvas, W, H, band, sat, lds, stage); break;
     case 2: sat_rows_body<SID, NT, E, G, 2 * G>(canvas, W, H, band, sat, lds, stage); break;
-    case 3: sat_rows_body<SID, NT, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
-    default: sat_rows_body<SID, NT, E, G, 4 * G>(canvas, W, H, band, sat, lds, stage); break;
+    default: sat_rows_body<SID, NT, E, G, 3 * G>(canvas, W, H, band, sat, lds, stage); break;
   }
 }
 
 int sat_bands(int H) { return (H + kSatBand - 1) / kSatBand; }
 
 template <int SID>
-static void sat_build_S(const float* canvas, int W, int H, double* band, double* sat, hipStream_t st) {
+static void sat_build_S(const float* canvas, int W, int H, double* band, double2* sat, hipStream_t st) {
   constexpr int n_tab = sat_set(SID).n;
   const int nb = sat_bands(H);
   k_sat_band<SID><<<dim3(nb, (W + 255) / 256), 256, 0, st>>>(canvas, W, H, band);
   k_sat_carry<<<(unsigned)(n_tab * ((W + 63) / 64)), 64 * kCarryWaves, 0, st>>>(band, W, nb);
-  constexpr int G = 3;   // tables per workgroup
+  constexpr int G = 4;   // tables per workgroup (two pairs)
   const dim3 g(nb, (n_tab + G - 1) / G);
   if (W <= 256 * 4) k_sat_rows<SID, 256, 4, G><<<g, 256, 0, st>>>(canvas, W, H, band, sat);
   else k_sat_rows<SID, 512, 4, G><<<g, 512, 0, st>>>(canvas, W, H, band, sat);
 }
 
-bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, double* band, double* sat,
+bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, double* band, double2* sat,
                       hipStream_t st) {
   if (W < 1 || H < 1 || W > 256 * 8) return false;
   // side A of a translation plan sums the same planes as its rectangle plan (plan − 4)
-  const int sid = side == 0 ? (plan >= 8 ? plan - 4 : plan) : 16 + plan;
+  const int sid = side == 2 ? kImageTables : side == 0 ? (plan >= 8 ? plan - 4 : plan) : 16 + plan;
   switch (sid) {
+    case kImageTables: sat_build_S<kImageTables>(canvas, W, H, band, sat, st); return true;
     case 4: sat_build_S<4>(canvas, W, H, band, sat, st); return true;
     case 5: sat_build_S<5>(canvas, W, H, band, sat, st); return true;
     case 6: sat_build_S<6>(canvas, W, H, band, sat, st); return true;
@@ -316,33 +319,38 @@ __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
   const int dx = px <= v.nx / 2 ? px : px - v.nx;
   const int dy = py <= v.ny / 2 ? py : py - v.ny;
   const int sx = v.sign * dx, sy = v.sign * dy;
-  const int x0 = min(max(v.bx0 + sx, 0), v.W), x1 = min(max(v.bx1 + sx, 0), v.W);
-  const int y0 = min(max(v.by0 + sy, 0), v.H), y1 = min(max(v.by1 + sy, 0), v.H);
-  const uint32_t SW = (uint32_t)v.W + 1;
+  const int x0 = min(max(v.bx0 + sx, v.cx0), v.cx1), x1 = min(max(v.bx1 + sx, v.cx0), v.cx1);
+  const int y0 = min(max(v.by0 + sy, v.cy0), v.cy1), y1 = min(max(v.by1 + sy, v.cy0), v.cy1);
+  const uint32_t SW = (uint32_t)v.stride;
   return {y0 * SW + x0, y0 * SW + x1, y1 * SW + x0, y1 * SW + x1};
 }
 
-// The sum over the rectangle from table k (base = k · table size, in elements; every table set
-// of a map stays below 2^32 bytes, so 32-bit element offsets address it).
-__device__ __forceinline__ double sat_sum(const double* __restrict__ S, uint32_t base, const SatIdx& q) {
-  return (S[base + q.o11] - S[base + q.o01]) - (S[base + q.o10] - S[base + q.o00]);
+// The sum over the rectangle from table k (pair k / 2, base = pair · table size in cells; every
+// table set of a map stays below 2^32 bytes, so 32-bit cell offsets address it). The two tables of
+// a pair load the same four cells, which the compiler merges into one load per corner.
+__device__ __forceinline__ double sat_sum(const double2* __restrict__ S, int k, uint32_t SP, const SatIdx& q) {
+  const uint32_t base = (uint32_t)(k >> 1) * SP;
+  const double2 a = S[base + q.o11], b = S[base + q.o01], c = S[base + q.o10], d = S[base + q.o00];
+  return (k & 1) ? (a.y - b.y) - (c.y - d.y) : (a.x - b.x) - (c.x - d.x);
 }
 
 // k_combine for the rectangle plans: correlations with sat_of >= 0 come from side A's tables, and
-// in translation plans those with bsat_of >= 0 from side B's.
+// in translation plans those with bsat_of >= 0 from side B's. A rectangle plan reads canvas A's
+// tables (table k: the plan's k-th sum); a translation plan reads each image's own tables (table
+// k: plane k), both at an offset (SatView).
 template <int PID>
 __global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ corr, float* map, int64_t nn,
                                                       CombineArgs a, SatView sv, SatView svb, float* arg_part) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr bool EXACT = PID & 1;
   constexpr bool TRANS = P.n_bsat > 0;
-  const uint32_t SP = (uint32_t)(sv.W + 1) * (uint32_t)(sv.H + 1);
+  const uint32_t SP = sv.cells, SPB = svb.cells;
   SatIdx q = sat_idx(sv, 0), qb = q;
   if constexpr (TRANS) qb = sat_idx(svb, 0);
   auto C = [&](int num, int64_t t) -> double {
-    if (P.sat_of[num] >= 0) return sat_sum(sv.sat, (uint32_t)P.sat_of[num] * SP, q);
+    if (P.sat_of[num] >= 0) return sat_sum(sv.sat, TRANS ? P.sat_plane[P.sat_of[num]] : P.sat_of[num], SP, q);
     if constexpr (TRANS)
-      if (P.bsat_of[num] >= 0) return sat_sum(svb.sat, (uint32_t)P.bsat_of[num] * SP, qb);
+      if (P.bsat_of[num] >= 0) return sat_sum(svb.sat, P.bsat_plane[P.bsat_of[num]], SPB, qb);
     return (double)corr[(int64_t)P.kept_of[num] * nn + t];
   };
   const double n0 = C(0, 0);

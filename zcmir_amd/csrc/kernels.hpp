@@ -75,17 +75,25 @@ int combine_blocks(int64_t nn);
 // Summed-area tables of the rectangle-sum planes of a rectangle plan (4..7): table k of plane
 // kPlans[plan].sat_plane[k].
 int sat_bands(int H);
-// canvas: 5 planes W x H; band: n_sat · sat_bands(H) · W doubles; sat: n_sat · (W+1)(H+1) doubles
-// side 0: side A's tables of a rectangle or translation plan; side 1: side B's (translation plans)
-bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, double* band, double* sat,
+// canvas: 5 planes W x H; band: n_sat · sat_bands(H) · W doubles; sat: sat_pairs(n_sat) ·
+// (W+1)(H+1) double2 (tables 2p, 2p+1 interleaved)
+// side 0: side A's tables of a rectangle plan (canvas A); side 2: an image's own 15 tables (its
+// stack as the canvas, W x H = the image)
+bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, double* band, double2* sat,
                       hipStream_t st);
-// The tables and the fixed rectangle [bx0, bx1) x [by0, by1) in canvas cells; nx = 1 << lx.
-// sign +1: the rectangle moves with the lag (side A's tables, R = the fixed rectangle: R + t);
-// -1: against it (side B's tables of a translation plan, R = the moving rectangle: R − t).
+inline constexpr int sat_pairs(int n_tables) { return (n_tables + 1) / 2; }
+// The tables and the rectangle [bx0, bx1) x [by0, by1) in table cells; nx = 1 << lx.
+// sign +1: the rectangle moves with the lag (side A's sums, R = the fixed rectangle: R + t);
+// -1: against it (side B's sums of a translation plan, R = the moving rectangle: R − t).
+// The moved rectangle is clipped to [cx0, cx1] x [cy0, cy1] (the canvas, or the part of an image's
+// own tables that lies on the canvas); a table row holds `stride` cells (its width + 1) and a
+// table `cells` cells ((width + 1) x (height + 1)).
 struct SatView {
-  const double* sat;
-  int W, H, nx, ny, lx, bx0, bx1, by0, by1;
+  const double2* sat;   // the table pairs (k_sat_rows' layout)
+  int nx, ny, lx, bx0, bx1, by0, by1;
   int sign;
+  int cx0, cx1, cy0, cy1, stride;
+  uint32_t cells;
 };
 // sv: side A's tables (rectangle plans); svb: side B's (translation plans)
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,

@@ -150,8 +150,17 @@ struct SatSet {
   int plane[15];
 };
 
+// sid < 16: side A's tables of plan sid; 16 + plan: side B's; kImageTables: an image's own tables,
+// table k of plane k for all 15 planes (translation plans read both sides' sums from these).
+inline constexpr int kImageTables = 32;
+
 constexpr SatSet sat_set(int sid) {
   SatSet t{};
+  if (sid == kImageTables) {
+    t.n = 15;
+    for (int k = 0; k < 15; ++k) t.plane[k] = k;
+    return t;
+  }
   const CorrPlan& p = kPlans[sid & 15];
   t.n = sid < 16 ? p.n_sat : p.n_bsat;
   for (int k = 0; k < 15; ++k) t.plane[k] = sid < 16 ? p.sat_plane[k] : p.bsat_plane[k];
