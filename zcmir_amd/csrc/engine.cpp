@@ -587,7 +587,8 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     ensure_tables(B);
   }
   work.ensure((size_t)N * 8 * pi.n_out);
-  corr.ensure((size_t)N * 4 * pi.n_corr);
+  // rectangle plans: correlation quads (k_rowifft_quads), else one plane per correlation
+  corr.ensure((size_t)N * 4 * (rect ? 4 * ((pi.n_corr + 3) / 4) : pi.n_corr));
   map.ensure((size_t)N * 4);
   const int nparts = combine_blocks(N);
   arg_part.ensure((size_t)nparts * 8);
@@ -640,7 +641,10 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   LAUNCH("cmul", launch_cmul(plan, spa, spb, work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
-  LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
+  if (rect)
+    LAUNCH("rowifft_extract", ok &= launch_rowifft_quads(plan, work.as<float2>(), corr.as<float4>(), nx, ny, twiddles(nx), st));
+  else
+    LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), nx, ny, twiddles(nx), st));
   if (!ok) throw std::runtime_error("UnsupportedFftLength");
   const uint32_t floor_n = min_overlap > 0 ? (uint32_t)std::floor(min_overlap * (double)cw * (double)ch) : 0u;
   CombineArgs ca{};

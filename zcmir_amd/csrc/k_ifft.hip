@@ -38,6 +38,60 @@ __global__ __launch_bounds__((RowX<L, V>::T)) void k_rowifft_extract(const float
 }
 
 
+// The rectangle plans' layout: correlation k in component k & 3 of quad plane k >> 2 (float4 per
+// lag), so the per-lag score reads four correlations per load. Workgroup (row y, quad p) inverts
+// outputs 2p and 2p+1 of row y together (two register groups per thread, one LDS line) and
+// stores the four planes' values of each lag as one float4; a missing second output stores 0.
+template <int L, int PID>
+__global__ __launch_bounds__(L / 16) void k_rowifft_quads(const float2* __restrict__ work, float4* corr4, int ny,
+                                                          const float2* __restrict__ tw) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr int M = 16, T = L / 16, STR = padded_len(L);
+  __shared__ float2 lds[STR];
+  const int tid = threadIdx.x, y = blockIdx.x, p = blockIdx.y;
+  const int q0 = 2 * p;
+  const bool two = q0 + 1 < P.n_out;
+  const int64_t N = (int64_t)L * ny;
+  const float2* r0 = work + (int64_t)q0 * N + (int64_t)y * L;
+  const float2* r1 = r0 + N;
+  float2 v[2][M];
+  rload<L, M, T, 1, false>(v[0], tid, [&](int, int x) { return r0[x]; });
+  rload<L, M, T, 1, false>(v[1], tid, [&](int, int x) { return two ? r1[x] : make_float2(0.0f, 0.0f); });
+  rfft_groups<L, M, T, 1, false, 1, 2>(v, lds, STR, tw, tid);
+  float4* out = corr4 + (int64_t)p * N + (int64_t)y * L;
+  int i = 0;   // rstore visits v[0][0], v[0][1], ... in order: v[1][i] is the same lag of output 2p + 1
+  rstore<L, M, T, 1, false>(v[0], tid, [&](int, int x, float2 a) {
+    const float2 b = v[1][i++];
+    out[x] = make_float4(a.x, a.y, b.x, b.y);
+  });
+}
+
+template <int L>
+static void rowifft_quads_L(int plan, const float2* work, float4* corr4, int ny, const float2* tw, hipStream_t st) {
+  {
+    const dim3 g(ny, (kPlans[plan].n_out + 1) / 2);
+    switch (plan) {
+      case 4: k_rowifft_quads<L, 4><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      case 5: k_rowifft_quads<L, 5><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      case 6: k_rowifft_quads<L, 6><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      case 7: k_rowifft_quads<L, 7><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      case 8: k_rowifft_quads<L, 8><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      case 9: k_rowifft_quads<L, 9><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      case 10: k_rowifft_quads<L, 10><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+      default: k_rowifft_quads<L, 11><<<g, L / 16, 0, st>>>(work, corr4, ny, tw); break;
+    }
+  }
+}
+
+bool launch_rowifft_quads(int plan, const float2* work, float4* corr4, int nx, int ny, const float2* tw_x,
+                          hipStream_t st) {
+  if (plan < 4) return false;
+#define CALL(LL) rowifft_quads_L<LL>(plan, work, corr4, ny, tw_x, st)
+  ZC_POW2_SWITCH(nx, CALL)
+#undef CALL
+  return true;
+}
+
 template <int L, int V>
 static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
   using X = RowX<L, V>;

@@ -339,7 +339,7 @@ __device__ __forceinline__ double sat_sum(const double2* __restrict__ S, int k, 
 // tables (table k: the plan's k-th sum); a translation plan reads each image's own tables (table
 // k: plane k), both at an offset (SatView).
 template <int PID>
-__global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ corr, float* map, int64_t nn,
+__global__ __launch_bounds__(256) void k_combine_rect(const float4* __restrict__ corr4, float* map, int64_t nn,
                                                       CombineArgs a, SatView sv, SatView svb, float* arg_part) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr bool EXACT = PID & 1;
@@ -351,7 +351,10 @@ __global__ __launch_bounds__(256) void k_combine_rect(const float* __restrict__ 
     if (P.sat_of[num] >= 0) return sat_sum(sv.sat, TRANS ? P.sat_plane[P.sat_of[num]] : P.sat_of[num], SP, q);
     if constexpr (TRANS)
       if (P.bsat_of[num] >= 0) return sat_sum(svb.sat, P.bsat_plane[P.bsat_of[num]], SPB, qb);
-    return (double)corr[(int64_t)P.kept_of[num] * nn + t];
+    // the four correlations of a quad load together (k_rowifft_quads' layout)
+    const int k = P.kept_of[num];
+    const float4 v = corr4[(int64_t)(k >> 2) * nn + t];
+    return (double)((k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w);
   };
   const double n0 = C(0, 0);
   const bool calib = EXACT && a.calib;
@@ -506,19 +509,21 @@ void launch_combine(int plan, const float* corr, float* map, int64_t nn, const C
   if (plan >= 4 && !sv) throw std::runtime_error("rectangle plan without tables");
   if (plan >= 8 && !svb) throw std::runtime_error("translation plan without the fixed side's tables");
   const SatView b = svb ? *svb : SatView{};
+  // rectangle plans: corr holds the correlation quads of launch_rowifft_quads
+  const float4* c4 = reinterpret_cast<const float4*>(corr);
   switch (plan) {
     case 0: k_combine<0><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 1: k_combine<1><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 2: k_combine<2><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
     case 3: k_combine<3><<<g, 256, 0, st>>>(corr, map, nn, a, arg_part); break;
-    case 4: k_combine_rect<4><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    case 5: k_combine_rect<5><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    case 6: k_combine_rect<6><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    case 7: k_combine_rect<7><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    case 8: k_combine_rect<8><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    case 9: k_combine_rect<9><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    case 10: k_combine_rect<10><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
-    default: k_combine_rect<11><<<g, 256, 0, st>>>(corr, map, nn, a, *sv, b, arg_part); break;
+    case 4: k_combine_rect<4><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    case 5: k_combine_rect<5><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    case 6: k_combine_rect<6><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    case 7: k_combine_rect<7><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    case 8: k_combine_rect<8><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    case 9: k_combine_rect<9><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    case 10: k_combine_rect<10><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
+    default: k_combine_rect<11><<<g, 256, 0, st>>>(c4, map, nn, a, *sv, b, arg_part); break;
   }
 }
 

@@ -61,6 +61,10 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
                  hipStream_t st, int batch = 1);
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
                             hipStream_t st, int batch = 1);
+// Rectangle and translation plans (4..11), lines of 32..4096: the correlations as quads
+// (correlation k: component k & 3 of float4 plane k >> 2, (n_corr + 3) / 4 planes of nx · ny).
+bool launch_rowifft_quads(int plan, const float2* work, float4* corr4, int nx, int ny, const float2* tw_x,
+                          hipStream_t st);
 // Maps at integer-translation poses (full resolution): 1 (default) the translation plans, 0 the
 // rectangle plans every other full-resolution pose takes; process-wide (zc_map_path)
 struct MapPath {
@@ -95,7 +99,8 @@ struct SatView {
   int cx0, cx1, cy0, cy1, stride;
   uint32_t cells;
 };
-// sv: side A's tables (rectangle plans); svb: side B's (translation plans)
+// sv: side A's tables (rectangle plans); svb: side B's (translation plans); rectangle plans read
+// corr in launch_rowifft_quads' layout, the others one plane per correlation
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
                     hipStream_t st, const SatView* sv = nullptr, const SatView* svb = nullptr);
 // out (ry x rx) = the lags of a wrapped map (ny x nx, nx >= rx, ny >= ry) on a grid of rx x ry:
