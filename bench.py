@@ -234,11 +234,19 @@ def bench_stack(a, np, torch, zcmir_amd, synth):
     if d and "tflops" in d:
         out["roofline"] = {"bound": "valu", "kernel": "pose_moments", "achieved": d["tflops"], "peak": FP32_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": round(d["tflops"] / FP32_PEAK_TFLOPS, 4),
-                           "flops_per_pixel_pose": POSE_MOMENT_FLOPS, "traffic": None}
+                           "flops_per_pixel_pose": POSE_MOMENT_FLOPS, "traffic": _traffic("stack", "pose_moments")}
     out["note"] = ("init_map: each start corrected by its shift map's peak (translation only; rotations up to "
                    "10 deg are left to Gauss-Newton); init_sweep: by a +-12 deg, 0.88-1.12 sweep's best peak. "
                    "Kernel rows come from one more registration (sweep starts) with HIP events around every launch.")
     return out
+
+
+def _traffic(section, kernel):
+    """HBM bytes per launch of a kernel from profiles/hbm_traffic.json (PMC, calibrated), or None."""
+    tfile = ROOT / "profiles" / "hbm_traffic.json"
+    if not tfile.exists():
+        return None
+    return json.loads(tfile.read_text()).get(section, {}).get(kernel)
 
 
 def calibrate_copy(mb):

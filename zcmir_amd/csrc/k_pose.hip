@@ -364,10 +364,13 @@ void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_
   k_pose_moments<<<moments_blocks(wb, hb), 256, 0, st>>>(feat_a, wa, ha, feat_b, wb, hb, p, part);
 }
 
-// Workgroups per pose: about 4 chunks per wave, at most 256.
+// Workgroups per pose: about 4 chunks per wave, at most 256, a multiple of 8 — the hardware deals
+// workgroups to the 8 XCDs in turn, so chunk x of consecutive poses (the tries of one slice, which
+// read the same features) runs on the same XCD and shares its L2.
 int jobs_blocks(int64_t max_pixels) {
   const int64_t chunks = (max_pixels + kChunkPx - 1) / kChunkPx;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(256, (chunks + 15) / 16));
+  const int64_t b = std::max<int64_t>(1, std::min<int64_t>(256, (chunks + 15) / 16));
+  return (int)((b + 7) / 8 * 8);
 }
 
 void launch_pose_moments_jobs(const PoseJob* jobs, int njobs, int blocks, double* part, double* sums,
