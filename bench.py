@@ -299,6 +299,7 @@ def bench_sweep(a, np, torch, zcmir_amd, synth):
         reg = zcmir_amd.Registrar(half=False, symmetric=True, **SWEEP_GRID, **kw)
         reg.set_images(mv, fx, preprocess=False)
         info, peaks, best = reg.sweep()
+        tuning = [e["text"] for e in reg.events() if e.get("kind") == "log" and "per batch" in e.get("text", "")]
         k = int(np.argmax(best))
         th_deg, sg = -180 + 5 * (k // 9), 1.25 ** ((k % 9 - 4) / 4)
         if th_deg != -35 or abs(sg - 1.25 ** -0.5) > 1e-9:
@@ -323,7 +324,7 @@ def bench_sweep(a, np, torch, zcmir_amd, synth):
             "value": round(info["nc"] * steps / wall, 2), "unit": "poses/s", "steps": steps,
             "ms_per_sweep": round(1e3 * wall / steps, 3),
             "ms_per_sweep_eager_profiled": round(1e3 * wall_eager / steps, 3),
-            "candidates": info["nc"], "grid": info["grid"],
+            "candidates": info["nc"], "grid": info["grid"], "batch_tuning": tuning,
             "cell_px": round(info["cell"], 4), "largest_fft": info["n"],
             "fft_column_tiling": {str(info["n"]): reg.fft_tiling(info["n"])},
             "roofline": _roofline(kernels),

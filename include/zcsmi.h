@@ -205,8 +205,9 @@ int32_t zc_sweep_cells(void* e, const double H0[9], const uint32_t* cells, uint3
                        zc_sweep_peak* peaks, size_t cap, float* best, size_t nbest);
 /* the score map of the last sweep batch's first candidate; dims = (nx, ny) */
 int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
-/* candidates per sweep batch (0: 256; the reference's batches hold 64); process-wide; results
- * do not depend on it */
+/* candidates per sweep batch (0: with the fft_autotune setting on, the size timed fastest per
+ * plane size, at most 256, else 256; the reference's batches hold 64); process-wide; results do
+ * not depend on it */
 void zc_sweep_tune(uint32_t batch);
 /* this library's: full-resolution maps at integer-translation poses from now on, 1 (default)
  * through the translation plans (the moving weight plane is a rectangle too: the correlations

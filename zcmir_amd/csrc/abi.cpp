@@ -80,6 +80,7 @@ zc::SweepOpts sweep_opts(Engine* e, zc::SweepGrid& gv) {
   o.n_peak = 4;
   const int tb = zc::SweepTune::batch;
   o.batch = tb > 0 ? (uint32_t)tb : zc::kSweepBatch;
+  o.tune_batch = tb <= 0 && s.b("fft_autotune");
   o.min_overlap = clampf(s.f("min_overlap"), 0, 0.5);
   return o;
 }
@@ -539,7 +540,7 @@ int32_t zc_sweep_map(void* p, float* map, size_t len, uint32_t* dims) {
   return guard(e, "sweep_map", [&] { e->read_sweep_map(map, len, dims); });
 }
 
-// Candidates per sweep batch (0: kSweepBatch, 256).
+// Candidates per sweep batch (0: tuned per plane size up to kSweepBatch when fft_autotune is on).
 void zc_sweep_tune(uint32_t batch) { zc::SweepTune::batch = (int)batch; }
 
 // Full-resolution maps at integer-translation poses from now on: 1 (default) through the

@@ -174,7 +174,8 @@ struct FftTune {
 // Candidates per sweep batch by default: 256 (the reference uses 64; a batch only schedules the
 // candidates' independent maps, and 256 measured 47.0K against 44.7K poses/s on the 128² canvas).
 inline constexpr uint32_t kSweepBatch = 256;
-// Sweep tuning: candidates per batch (0: kSweepBatch).
+// Sweep tuning: candidates per batch (0: the timed best per plane size up to kSweepBatch when the
+// fft_autotune setting is on, else kSweepBatch).
 struct SweepTune {
   static inline std::atomic<int> batch{0};
   static inline std::atomic<int> budget_gb{32};   // work buffers of one batch, GiB
@@ -187,7 +188,8 @@ struct SweepOpts {
   bool full = false;     // one cell per fixed pixel (grid = ceil(D))
   uint32_t mode = 1;     // 0 global, 1 exact SMI
   uint32_t n_peak = 4;
-  uint32_t batch = kSweepBatch;
+  uint32_t batch = kSweepBatch;   // the most candidates per batch
+  bool tune_batch = false;        // time batch sizes up to `batch` per plane size on first use
   double min_overlap = 0.05;
   // only these (rotation, scale) index pairs of the grid (shear and stretch index 0), in order
   const std::vector<std::array<uint32_t, 2>>* cells = nullptr;

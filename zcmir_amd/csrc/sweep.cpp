@@ -12,6 +12,9 @@
 #include <cstring>
 #include <numeric>
 
+#include <map>
+#include <mutex>
+
 #include "engine.hpp"
 #include "plan.hpp"
 
@@ -204,6 +207,119 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   ca.ridge = kRidge;
   ca.area = 1.0;
   const SweepPose* dposes = sw_poses.as<SweepPose>();
+  // the fixed planes of a group's plane size: its footprint at the plane origin
+  auto fixed_batch = [&](const Group& gr) {
+    bool ok = true;
+    const uint32_t nx = gr.nx, ny = gr.ny;
+    const float2* tx = twiddles((int)nx);
+    const float2* ty = twiddles((int)ny);
+    SweepPack uf{(float)ox, (float)oy, (float)cell, (int)grid, B.w, B.h, (int)Fx, (int)Fy, (int)ny};
+    const double NCd = pi.n_cplx, nnd = (double)nx * ny;
+    bytes("pack_sweep_fixed", NCd * Fy * nx * 8 + 16.0 * B.w * B.h);
+    bytes("colfft_fwd_fixed", NCd * (Fy * nx * 8.0 + nnd * 8));
+    LAUNCH("pack_sweep_fixed", ok &= launch_pack_sweep(plan, B.il.as<float4>(), sw_pose_fixed.as<SweepPose>(), 1, uf,
+                                                        (int)nx, spec_b.as<float2>(), tx, st));
+    LAUNCH("colfft_fwd_fixed", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, (int)nx, (int)ny, (int)Fy, false,
+                                                    ty, st, col_lines((int)ny)));
+    if (!ok) throw std::runtime_error("UnsupportedFftLength");
+  };
+  // b candidates from slot c0 of a group: moving windows (as wide as the plane allows without
+  // wraparound), spectra, cross spectra, inverse transforms, scores, peaks
+  auto moving_batch = [&](const Group& gr, uint32_t c0, int b) {
+    bool ok = true;
+    const uint32_t nx = gr.nx, ny = gr.ny;
+    const int64_t nn = (int64_t)nx * ny;
+    const float2* tx = twiddles((int)nx);
+    const float2* ty = twiddles((int)ny);
+    SweepPack um{(float)ox, (float)oy, (float)cell, (int)grid, A.w, A.h, (int)(nx - Fx + 1), (int)(ny - Fy + 1), (int)ny};
+    const double NCd = pi.n_cplx, Qd = pi.n_out, Cd = pi.n_corr, nnd = (double)nn;
+    // algorithmic bytes of each launch: its role's minimal HBM traffic
+    bytes("pack_sweep", b * (NCd * um.win_y * nx * 8 + 16.0 * A.w * A.h));
+    bytes("colfft_fwd", b * NCd * (um.win_y * nx * 8.0 + nnd * 8));
+    bytes("cmul", b * (2 * NCd + Qd) * nnd * 8);
+    bytes("colfft_inv", b * 2 * Qd * nnd * 8);
+    bytes("rowifft_extract", b * (Qd * nnd * 8 + Cd * nnd * 4));
+    // every correlation plane at every lag (lags under the floor read only n, but their cache
+    // lines hold lags above it) and the score written
+    bytes("combine_sweep", b * (Cd + 1) * nnd * 4);
+    bytes("peaks", b * nnd * 4);
+    LAUNCH("pack_sweep", ok &= launch_pack_sweep(plan, A.il.as<float4>(), dposes + c0, b, um, (int)nx,
+                                                  spec_a.as<float2>(), tx, st));
+    LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx * b, (int)nx, (int)ny, um.win_y, false,
+                                            ty, st, col_lines((int)ny)));
+    LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), (int)nx, (int)ny,
+                               st, b));
+    LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out * b, (int)nx, (int)ny, (int)ny, true, ty,
+                                            st, col_lines((int)ny)));
+    LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), (int)nx,
+                                                           (int)ny, tx, st, b));
+    LAUNCH("combine_sweep", launch_combine_sweep(plan, corr.as<float>(), sw_ns.as<float>(), nn, b, ca,
+                                                 sw_zero.as<int>() + c0, st));
+    LAUNCH("peaks", launch_peaks(sw_ns.as<float>(), (int)nx, (int)ny, b, (int)n_peak, sw_part.as<float2>(),
+                                 sw_peaks.as<float2>() + (size_t)c0 * n_peak, st));
+    if (!ok) throw std::runtime_error("UnsupportedFftLength");
+  };
+  // Candidates per batch, tuned per (plan, plane size, window) on first use unless forced: each
+  // power of two up to the memory bound (and the bound itself) runs one batch untimed, then the
+  // group's first max(4 b, 16) candidates (at most the group) in batches of b, timed twice; the
+  // least time per candidate wins (within 3 %, the smaller batch). The runs write the same outputs
+  // the sweep then writes again.
+  auto tuned_batch = [&](const Group& gr) -> uint32_t {
+    static std::mutex mu;
+    static std::map<std::array<uint32_t, 6>, uint32_t> best;
+    const std::array<uint32_t, 6> k{(uint32_t)plan, gr.nx, gr.ny, Fx, Fy, gr.bn};
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      const auto it = best.find(k);
+      if (it != best.end()) return it->second;
+    }
+    const uint32_t n_grp = gr.k1 - gr.k0, avail = std::min(gr.bn, n_grp);
+    std::vector<uint32_t> sizes;
+    for (uint32_t b = 1; b < avail; b *= 2) sizes.push_back(b);
+    sizes.push_back(avail);
+    hipEvent_t e0, e1;
+    ZC_HIP(hipEventCreate(&e0));
+    ZC_HIP(hipEventCreate(&e1));
+    fixed_batch(gr);
+    double t_best = INFINITY;
+    uint32_t b_best = gr.bn;
+    for (const uint32_t b : sizes) {
+      const uint32_t n = std::min(n_grp, std::max(4 * b, 16u));
+      moving_batch(gr, gr.k0, (int)b);
+      float t = INFINITY;
+      for (int r = 0; r < 2; ++r) {
+        ZC_HIP(hipEventRecord(e0, st));
+        for (uint32_t c0 = gr.k0; c0 < gr.k0 + n; c0 += b) moving_batch(gr, c0, (int)std::min(b, gr.k0 + n - c0));
+        ZC_HIP(hipEventRecord(e1, st));
+        ZC_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        ZC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        t = std::min(t, ms);
+      }
+      const double per = (double)t / n;
+      if (per < 0.97 * t_best) {
+        t_best = per;
+        b_best = b;
+      }
+    }
+    ZC_HIP(hipEventDestroy(e0));
+    ZC_HIP(hipEventDestroy(e1));
+    prof.break_chain();
+    events.log("sweep: %u candidates per batch at %ux%u planes (timed, %.3f ms per candidate)", b_best, gr.nx, gr.ny,
+               t_best);
+    std::lock_guard<std::mutex> lk(mu);
+    best[k] = b_best;
+    return b_best;
+  };
+  if (o.tune_batch && !prof.on)
+    for (Group& gr : groups)
+      if (gr.k1 - gr.k0 > 1) gr.bn = tuned_batch(gr);
+  auto issue = [&] {
+    for (const Group& gr : groups) {
+      fixed_batch(gr);
+      for (uint32_t c0 = gr.k0; c0 < gr.k1; c0 += gr.bn) moving_batch(gr, c0, (int)std::min(gr.bn, gr.k1 - c0));
+    }
+  };
   // everything the launches read from the host, resolved before a capture: twiddles and the
   // column tilings (which may time the tilings on first use), then the key of the launch sequence
   std::vector<uint8_t> key;
@@ -231,54 +347,6 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   for (const void* p : {A.il.p, B.il.p, spec_a.p, spec_b.p, work.p, corr.p, sw_ns.p, sw_part.p, sw_peaks.p,
                         sw_poses.p, sw_pose_fixed.p, sw_zero.p})
     put(p);
-  auto issue = [&] {
-  bool ok = true;
-  for (const Group& gr : groups) {
-    const uint32_t k0 = gr.k0, k1 = gr.k1, nx = gr.nx, ny = gr.ny, Bn = gr.bn;
-    const int64_t nn = (int64_t)nx * ny;
-    const float2* tx = twiddles((int)nx);
-    const float2* ty = twiddles((int)ny);
-    // the fixed planes: its footprint at the plane origin
-    SweepPack uf{(float)ox, (float)oy, (float)cell, (int)grid, B.w, B.h, (int)Fx, (int)Fy, (int)ny};
-    const double NCd = pi.n_cplx, Qd = pi.n_out, Cd = pi.n_corr, nnd = (double)nn;
-    bytes("pack_sweep_fixed", NCd * Fy * nx * 8 + 16.0 * B.w * B.h);
-    bytes("colfft_fwd_fixed", NCd * (Fy * nx * 8.0 + nnd * 8));
-    LAUNCH("pack_sweep_fixed", ok &= launch_pack_sweep(plan, B.il.as<float4>(), sw_pose_fixed.as<SweepPose>(), 1, uf,
-                                                        (int)nx, spec_b.as<float2>(), tx, st));
-    LAUNCH("colfft_fwd_fixed", ok &= launch_colfft(spec_b.as<float2>(), pi.n_cplx, (int)nx, (int)ny, (int)Fy, false,
-                                                    ty, st, col_lines((int)ny)));
-    // moving windows: as wide as the plane allows without wraparound
-    SweepPack um{(float)ox, (float)oy, (float)cell, (int)grid, A.w, A.h, (int)(nx - Fx + 1), (int)(ny - Fy + 1), (int)ny};
-    for (uint32_t c0 = k0; c0 < k1; c0 += Bn) {
-      const int b = (int)std::min(Bn, k1 - c0);
-      // algorithmic bytes of each launch: its role's minimal HBM traffic
-      bytes("pack_sweep", b * (NCd * um.win_y * nx * 8 + 16.0 * A.w * A.h));
-      bytes("colfft_fwd", b * NCd * (um.win_y * nx * 8.0 + nnd * 8));
-      bytes("cmul", b * (2 * NCd + Qd) * nnd * 8);
-      bytes("colfft_inv", b * 2 * Qd * nnd * 8);
-      bytes("rowifft_extract", b * (Qd * nnd * 8 + Cd * nnd * 4));
-      // every correlation plane at every lag (lags under the floor read only n, but their cache
-      // lines hold lags above it) and the score written
-      bytes("combine_sweep", b * (Cd + 1) * nnd * 4);
-      bytes("peaks", b * nnd * 4);
-      LAUNCH("pack_sweep", ok &= launch_pack_sweep(plan, A.il.as<float4>(), dposes + c0, b, um, (int)nx,
-                                                    spec_a.as<float2>(), tx, st));
-      LAUNCH("colfft_fwd", ok &= launch_colfft(spec_a.as<float2>(), pi.n_cplx * b, (int)nx, (int)ny, um.win_y, false,
-                                              ty, st, col_lines((int)ny)));
-      LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), (int)nx, (int)ny,
-                                 st, b));
-      LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out * b, (int)nx, (int)ny, (int)ny, true, ty,
-                                              st, col_lines((int)ny)));
-      LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), (int)nx,
-                                                             (int)ny, tx, st, b));
-      LAUNCH("combine_sweep", launch_combine_sweep(plan, corr.as<float>(), sw_ns.as<float>(), nn, b, ca,
-                                                   sw_zero.as<int>() + c0, st));
-      LAUNCH("peaks", launch_peaks(sw_ns.as<float>(), (int)nx, (int)ny, b, (int)n_peak, sw_part.as<float2>(),
-                                   sw_peaks.as<float2>() + (size_t)c0 * n_peak, st));
-      if (!ok) throw std::runtime_error("UnsupportedFftLength");
-    }
-  }
-  };
   run_captured(sweep_gc, std::move(key), prof.on || !set.b("sweep_graph"), issue);
   sweep_last_nx = groups.back().nx;
   sweep_last_ny = groups.back().ny;
