@@ -17,8 +17,8 @@ OBJ = HERE / "_build"
 LIB = HERE / "libzcsmi.so"
 ARCH = os.environ.get("ZCSMI_ARCH", "gfx950")
 
-SOURCES = ["k_prep.hip", "k_pack.hip", "k_ifft.hip", "k_score.hip", "k_fft.hip", "k_pose.hip", "k_sweep.hip", "noalloc.hip", "cnn.hip", "pre.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "rs.cpp", "climb.cpp", "abi.cpp"]
-HEADERS = ["kdev.hpp", "events.hpp", "fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
+SOURCES = ["k_prep.hip", "k_pack.hip", "k_ifft.hip", "k_score.hip", "k_fft.hip", "k_pose.hip", "k_sweep.hip", "k_gn.hip", "noalloc.hip", "cnn.hip", "pre.hip", "engine.cpp", "sweep.cpp", "gn.cpp", "stack.cpp", "rs.cpp", "climb.cpp", "abi.cpp"]
+HEADERS = ["kdev.hpp", "gn_dev.hpp", "events.hpp", "fft.hpp", "fft_regs.hpp", "plan.hpp", "kernels.hpp", "engine.hpp", "settings.hpp", "geometry.hpp",
            "search.hpp", "lie.hpp", "pre.hpp"]
 
 FLAGS = [

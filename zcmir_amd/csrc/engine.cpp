@@ -105,9 +105,11 @@ Engine::~Engine() {
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
                   &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
                   &gn_sums, &sat_band, &sat, &rs_fwd, &rs_inv, &rs_out, &rs_tab, &pre_buf, &pre_hist, &pre_cdf,
-                  &pre_part, &pre_rgba, &map_out})
+                  &pre_part, &pre_rgba, &map_out, &gd_pairs, &gd_state, &gd_gjobs, &gd_gsums, &gd_tjobs, &gd_tsums,
+                  &gd_dir, &gd_step, &gd_try_H, &gd_try_ok, &gd_try_jobs, &gd_try_sums, &gd_try_score, &gd_part, &gd_ctr})
     b->release();
   sweep_gc.reset();
+  gn_gc.reset();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
   if (ev_fork) (void)hipEventDestroy(ev_fork);

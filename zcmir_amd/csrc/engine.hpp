@@ -321,6 +321,7 @@ class Engine {
     void reset();
   };
   GraphCache sweep_gc;
+  GraphCache gn_gc;   // one Gauss-Newton iteration of the device batch
   // Issues `issue` (kernel launches only on st: no copies, fills, syncs or allocations) through
   // gc's graph: replayed when the key matches, else captured, instantiated and launched. Runs
   // `issue` eagerly when `eager` (profiling needs per-launch events).
@@ -340,6 +341,11 @@ class Engine {
   std::vector<GnOut> gn_batch(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, uint32_t iters,
                               bool homography, bool exact, bool symmetric, const std::vector<double>& amps,
                               uint32_t* iterations = nullptr);
+  // the same batch with every iteration on the device: one captured graph replayed per iteration,
+  // the state read back once at the end
+  std::vector<GnOut> gn_batch_device(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, uint32_t iters,
+                                     bool homography, bool exact, bool symmetric, const std::vector<double>& amps,
+                                     uint32_t* iterations);
   std::vector<PairSides> own_pair(size_t n) const;
   // best_peak: the sweep's best raw peak score (n in fixed pixels), nullable
   SearchResult search(const SweepOpts& o, uint32_t keep, uint32_t n_fin, bool homography, bool exact,
@@ -379,6 +385,9 @@ class Engine {
   DBuf pre_buf, pre_hist, pre_cdf, pre_part, pre_rgba;   // preprocessing scratch
   DBuf sw_poses, sw_pose_fixed, sw_zero, sw_ns, sw_part, sw_peaks;
   DBuf gn_jobs, gn_part, gn_sums;
+  // the device Gauss-Newton batch (GnBuffers)
+  DBuf gd_pairs, gd_state, gd_gjobs, gd_gsums, gd_tjobs, gd_tsums, gd_dir, gd_step, gd_try_H, gd_try_ok, gd_try_jobs,
+      gd_try_sums, gd_try_score, gd_part, gd_ctr;
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
   // Column-FFT tiling for column length L (launch_colfft's `lines`): zc_fft_tune's value when set,

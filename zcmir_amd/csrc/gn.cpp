@@ -385,6 +385,8 @@ std::vector<GradOut> Engine::pair_grads(const std::vector<PairSides>& pr, const 
 std::vector<GnOut> Engine::gn_batch(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs, uint32_t iters,
                                     bool homography, bool exact, bool symmetric, const std::vector<double>& amps,
                                     uint32_t* iterations) {
+  if (set.b("gn_graph") && !Hs.empty() && amps.size() <= 12)
+    return gn_batch_device(pr, Hs, iters, homography, exact, symmetric, amps, iterations);
   const size_t n = Hs.size();
   const int nk = n_keys(homography);
   std::vector<GnOut> cur(n);
@@ -458,6 +460,93 @@ std::vector<GnOut> Engine::gn_batch(const std::vector<PairSides>& pr, const std:
   }
   if (iterations) *iterations = it;
   return cur;
+}
+
+// gnBatch with every iteration on the device (k_gn.hip): the pairs and start poses go up once,
+// one captured graph (gradient sums, coefficients, tangent pass, tries, their sums and scores, the
+// selection) is replayed per iteration without a host wait, and the poses, scores and the count
+// of iterations come back once. Same arithmetic as the host loop above.
+std::vector<GnOut> Engine::gn_batch_device(const std::vector<PairSides>& pr, const std::vector<Mat3>& Hs,
+                                           uint32_t iters, bool homography, bool exact, bool symmetric,
+                                           const std::vector<double>& amps, uint32_t* iterations) {
+  const size_t n = Hs.size();
+  std::vector<GnPair> pairs(n);
+  std::vector<GnState> state(n);
+  int64_t px = 1;
+  for (size_t i = 0; i < n; ++i) {
+    const Side& a = *pr[i].a;
+    const Side& b = *pr[i].b;
+    ensure_il(a);
+    ensure_il(b);
+    pairs[i] = {a.il.as<float4>(), b.il.as<float4>(), a.w, a.h, b.w, b.h};
+    GnState& s = state[i];
+    for (int k = 0; k < 9; ++k) s.H[k] = Hs[i][k];
+    s.score = -INFINITY;
+    s.active = 1;
+    s.moved = 0;
+    px = std::max<int64_t>(px, std::max<int64_t>((int64_t)a.w * a.h, (int64_t)b.w * b.h));
+  }
+  GnArgs ga{};
+  ga.n = (int)n;
+  ga.homography = homography;
+  ga.exact = exact;
+  ga.symmetric = symmetric;
+  ga.min_n = kMinN;
+  ga.n_amps = (int)amps.size();
+  ga.ridge = kRidge;
+  for (size_t k = 0; k < amps.size(); ++k) ga.amps[k] = amps[k];
+  ga.n_try = (int)amps.size() + 5;
+  const int nd = (symmetric ? 2 : 1) * (int)n, T = ga.n_try;
+  const int blocks = jobs_blocks(px);
+  gd_pairs.ensure(n * sizeof(GnPair));
+  gd_state.ensure(n * sizeof(GnState));
+  gd_gjobs.ensure(nd * sizeof(PoseJob));
+  gd_gsums.ensure(nd * 45 * 8);
+  gd_tjobs.ensure(nd * sizeof(TangentJob));
+  gd_tsums.ensure(nd * 44 * 8);
+  gd_dir.ensure(2 * nd * 8);
+  gd_step.ensure(n * 18 * 8);
+  gd_try_H.ensure(n * T * 9 * 8);
+  gd_try_ok.ensure(n * T * 4);
+  gd_try_jobs.ensure((size_t)nd * T * sizeof(PoseJob));
+  gd_try_sums.ensure((size_t)nd * T * 45 * 8);
+  gd_try_score.ensure((size_t)nd * T * 8);
+  gd_part.ensure((size_t)nd * T * blocks * 45 * 8);
+  gd_ctr.ensure(2 * 4);
+  ZC_HIP(hipMemcpyAsync(gd_pairs.p, pairs.data(), n * sizeof(GnPair), hipMemcpyHostToDevice, st));
+  ZC_HIP(hipMemcpyAsync(gd_state.p, state.data(), n * sizeof(GnState), hipMemcpyHostToDevice, st));
+  ZC_HIP(hipMemsetAsync(gd_ctr.p, 0, 2 * 4, st));   // outside the graph: the counters start at 0
+  prof.break_chain();
+  GnBuffers gb{gd_pairs.as<GnPair>(), gd_state.as<GnState>(), gd_gjobs.as<PoseJob>(), gd_gsums.as<double>(),
+               gd_tjobs.as<TangentJob>(), gd_tsums.as<double>(), gd_dir.as<double>(), gd_dir.as<double>() + nd,
+               gd_step.as<double>(), gd_try_H.as<double>(), gd_try_ok.as<int>(), gd_try_jobs.as<PoseJob>(),
+               gd_try_sums.as<double>(),
+               gd_try_score.as<double>(), gd_part.as<double>(), gd_ctr.as<int>(), gd_ctr.as<int>() + 1};
+  std::vector<uint8_t> key;
+  auto put = [&key](const auto& v) {
+    const auto* p = reinterpret_cast<const uint8_t*>(&v);
+    key.insert(key.end(), p, p + sizeof v);
+  };
+  put(gb);
+  put(ga);
+  put(blocks);
+  for (uint32_t it = 0; it < iters && !cancel; ++it)
+    run_captured(gn_gc, std::vector<uint8_t>(key), prof.on,
+                 [&] { LAUNCH("gn_iteration", launch_gn_iteration(gb, ga, blocks, blocks, st)); });
+  int done = 0;
+  ZC_HIP(hipMemcpyAsync(state.data(), gd_state.p, n * sizeof(GnState), hipMemcpyDeviceToHost, st));
+  ZC_HIP(hipMemcpyAsync(&done, gd_ctr.as<int>() + 1, 4, hipMemcpyDeviceToHost, st));
+  prof.break_chain();
+  ++counts[2];
+  sync();
+  if (iterations) *iterations = (uint32_t)done;
+  std::vector<GnOut> out(n);
+  for (size_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 9; ++k) out[i].H[k] = state[i].H[k];
+    out[i].score = state[i].score;
+    out[i].moved = state[i].moved != 0;
+  }
+  return out;
 }
 
 std::vector<PairSides> Engine::own_pair(size_t n) const { return std::vector<PairSides>(n, {&sides[0], &sides[1]}); }

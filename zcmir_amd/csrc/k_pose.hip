@@ -208,6 +208,9 @@ __global__ __launch_bounds__(256) void k_pose_moments(const float* __restrict__ 
 // One pose per blockIdx.y: its partials at part[(y · gridDim.x + x) · 45].
 __global__ __launch_bounds__(256) void k_pose_moments_jobs(const PoseJob* __restrict__ jobs, double* part) {
   const PoseJob& j = jobs[blockIdx.y];
+  // an idle job (a slot of a device batch with nothing to sum; images are at least 8 px) leaves its
+  // partial sums unwritten: nothing reads them
+  if (j.wa < 5 || j.ha < 5) return;
   const PackedFeat A{j.fa}, B{j.fb};
   pose_moments_body(A, j.wa, j.ha, B, j.wb, j.hb, j.Hi, part + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 45,
                     blockIdx.x, gridDim.x);
@@ -329,6 +332,7 @@ __device__ __forceinline__ void tangent_pixel(const TangentJob& j, int64_t na, i
 
 __global__ __launch_bounds__(256) void k_tangent_jobs(const TangentJob* __restrict__ jobs, double* part) {
   const TangentJob& j = jobs[blockIdx.y];
+  if (j.wa < 5 || j.ha < 5) return;   // an idle job: nothing reads its partial sums
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wa = j.wa, ha = j.ha, wb = j.wb;
   const int64_t na = (int64_t)wa * ha, nb = (int64_t)wb * j.hb;

@@ -164,6 +164,48 @@ struct TangentJob {
   float Mk[72];
   float W[16], Mh[16], Q[16], ma[4], mb[4], g[45];
 };
+// ── Gauss–Newton batches on the device (k_gn.hip) ──
+inline constexpr int kGnMaxTries = 17;   // the ascent ladder (at most 12) and 5 step lengths
+// a pose's image pair: moving side a, fixed side b (interleaved features)
+struct GnPair {
+  const float4* fa;
+  const float4* fb;
+  int wa, ha, wb, hb;
+};
+// a pose in the batch: its current pose and score; active while it moves
+struct GnState {
+  double H[9];
+  double score;
+  int active, moved;
+};
+struct GnArgs {
+  int n, homography, exact, symmetric, min_n, n_amps;
+  int n_try;   // try slots per pose: n_amps + 5 (at most kGnMaxTries)
+  double ridge;
+  double amps[12];
+};
+// device buffers of a batch of n poses (nd = n, or 2n when symmetric; T = GnArgs::n_try)
+struct GnBuffers {
+  const GnPair* pairs;
+  GnState* state;
+  PoseJob* gjobs;        // nd
+  double* gsums;         // nd · 45
+  TangentJob* tjobs;     // nd
+  double* tsums;         // nd · 44
+  double* dir_score;     // nd
+  double* dir_n;         // nd
+  double* step;          // n · 18: unit ascent direction, Gauss-Newton step, its norm, flags
+  double* try_H;         // n · T · 9
+  int* try_ok;           // n · T: the slot holds a try
+  PoseJob* try_jobs;     // nd · T
+  double* try_sums;      // nd · T · 45
+  double* try_score;     // nd · T
+  double* part;          // nd · T · blocks · 45
+  int* moved_flag;       // 1
+  int* iters_done;       // 1
+};
+// one iteration for every active pose (kernel launches only: capturable)
+void launch_gn_iteration(const GnBuffers& b, const GnArgs& a, int blocks_grad, int blocks_try, hipStream_t st);
 int jobs_blocks(int64_t max_pixels);
 // 45 sums per job into sums (part: njobs · blocks · 45 doubles)
 void launch_pose_moments_jobs(const PoseJob* jobs, int njobs, int blocks, double* part, double* sums,
