@@ -69,10 +69,12 @@ def _plan(three, rect, trans=False):
     correlations with the fixed weight plane come from summed-area tables; translation plans:
     those with the moving weight plane too, from the fixed side's tables)."""
     if trans:
+        # the moving image's weight sums are the rectangles' areas: both images read the tables
+        # of their features and feature products only
         nf = 3 if three else 4
         C = nf * nf
         return {"n_cplx_a": (nf + 1) // 2, "n_cplx_b": (nf + 1) // 2, "n_out": (C + 1) // 2, "n_corr": C,
-                "n_sat": 10 if three else 15, "n_bsat": 9 if three else 14, "n_real_a": nf, "n_real_b": nf}
+                "n_sat": 9 if three else 14, "n_bsat": 9 if three else 14, "n_real_a": nf, "n_real_b": nf}
     if not rect:
         R = 10 if three else 15
         C = 28 if three else 45

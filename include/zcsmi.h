@@ -211,8 +211,10 @@ int32_t zc_sweep_map(void* e, float* map, size_t len, uint32_t dims[2]);
 void zc_sweep_tune(uint32_t batch);
 /* this library's: full-resolution maps at integer-translation poses from now on, 1 (default)
  * through the translation plans (the moving weight plane is a rectangle too: the correlations
- * with it come from summed-area tables, 16 of 45 through the FFT), 0 through the rectangle plans;
- * process-wide; the two agree within FP32 rounding */
+ * with it come from summed-area tables, 16 of 45 through the FFT), 0 through the rectangle plans,
+ * 2 through the translation plans with every lag reading its own four table cells per sum (1
+ * shares the cells of four lags where both images have the same size and the plane is twice as
+ * large; 1 and 2 give identical maps); process-wide; 0 and 1 agree within FP32 rounding */
 void zc_map_path(uint32_t translation_plans);
 
 /* stacks of uneven slice pairs (the role of python/zcmir/api.py:522-580 register_stack, batched):

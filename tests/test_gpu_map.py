@@ -331,3 +331,35 @@ def test_translation_plans_match_oracle_and_rectangle_plans(reg_factory, size, H
         lib.zc_map_path(1)
     assert (rect.dx, rect.dy) == (sm.dx, sm.dy)
     assert np.abs(rect.map - sm.map).max() <= MAP_TOL * np.abs(rect.map).max()
+
+
+@pytest.mark.parametrize("size,H,exact,mode,calibrated", [((1024, 1024), (0.0, 0.0), True, "laplacian", False),
+                                                          ((256, 256), (0.0, 0.0), True, "three", False),
+                                                          ((256, 256), (0.0, 0.0), False, "laplacian", False),
+                                                          ((256, 256), (0.0, 0.0), True, "laplacian", True),
+                                                          ((256, 256), (3.0, -2.0), True, "laplacian", False),
+                                                          ((200, 256), (0.0, 0.0), True, "laplacian", False)])
+def test_translation_plan_quad_cells_bit_identical(reg_factory, size, H, exact, mode, calibrated):
+    """Where both images have the same size and the FFT plane is twice as large, the translation
+    plans' score kernel loads the summed-area cells of four lags once (zc_map_path(1), the
+    default); every lag reading its own four cells per sum (zc_map_path(2)) gives the same map bit
+    for bit, also where the quad form does not apply (a shifted pose, unequal sizes)."""
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    mv, fx, _ = _pair(size, (5, -3), seed=7 + size[1], quantize=size[0] >= 1024)
+    Hp = np.array([[1, 0, H[0]], [0, 1, H[1]], [0, 0, 1.0]])
+    cap = 2048 if size[0] >= 1024 else 512
+    maps = []
+    try:
+        for path in (1, 2):
+            lib.zc_map_path(path)
+            reg = reg_factory(half=False, smi_features=mode)
+            reg.set_images(mv, fx, preprocess=False)
+            maps.append(reg.shift_map(Hp, exact=exact, calibrated=calibrated, resolution=cap))
+            reg.close()
+    finally:
+        lib.zc_map_path(1)
+    a, b = maps
+    assert (a.dx, a.dy) == (b.dx, b.dy)
+    assert a.map.tobytes() == b.map.tobytes()

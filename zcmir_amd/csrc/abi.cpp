@@ -546,7 +546,7 @@ void zc_sweep_tune(uint32_t batch) { zc::SweepTune::batch = (int)batch; }
 // Full-resolution maps at integer-translation poses from now on: 1 (default) through the
 // translation plans (both weight planes rectangles: 16 of 45 correlations by FFT), 0 through the
 // rectangle plans (30 by FFT). The two agree within FP32 rounding; process-wide.
-void zc_map_path(uint32_t translation_plans) { zc::MapPath::trans = translation_plans ? 1 : 0; }
+void zc_map_path(uint32_t translation_plans) { zc::MapPath::trans = translation_plans > 2 ? 1 : (int)translation_plans; }
 
 
 // ── reference entry points outside the SMI hot path: present so a ctypes binding of the

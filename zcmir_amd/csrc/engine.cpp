@@ -128,8 +128,8 @@ void Engine::release_side(Side& s) {
 // The image's own summed-area tables of its 15 stack planes (translation plans' rectangle sums).
 void Engine::ensure_tables(const Side& sd) {
   if (sd.tab_ok) return;
-  sd.tab.ensure((size_t)sat_pairs(15) * (sd.w + 1) * (sd.h + 1) * 16);
-  sat_band.ensure((size_t)15 * sat_bands(sd.h) * sd.w * 8);
+  sd.tab.ensure((size_t)sat_pairs(kImageTableCount) * (sd.w + 1) * (sd.h + 1) * 16);
+  sat_band.ensure((size_t)kImageTableCount * sat_bands(sd.h) * sd.w * 8);
   bool ok = true;
   LAUNCH("image_tables", ok = launch_sat_build(0, 2, sd.stack.as<float>(), sd.w, sd.h, sat_band.as<double>(),
                                                sd.tab.as<double2>(), st));

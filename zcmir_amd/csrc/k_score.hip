@@ -312,6 +312,7 @@ bool launch_sat_build(int plan, int side, const float* canvas, int W, int H, dou
 // read unconditionally (no branches; the zero row and column stay in the caches).
 struct SatIdx {
   uint32_t o00, o01, o10, o11;
+  int area;   // (x1 − x0)(y1 − y0): the rectangle's cell count
 };
 
 __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
@@ -322,7 +323,7 @@ __device__ __forceinline__ SatIdx sat_idx(const SatView& v, int64_t t) {
   const int x0 = min(max(v.bx0 + sx, v.cx0), v.cx1), x1 = min(max(v.bx1 + sx, v.cx0), v.cx1);
   const int y0 = min(max(v.by0 + sy, v.cy0), v.cy1), y1 = min(max(v.by1 + sy, v.cy0), v.cy1);
   const uint32_t SW = (uint32_t)v.stride;
-  return {y0 * SW + x0, y0 * SW + x1, y1 * SW + x0, y1 * SW + x1};
+  return {y0 * SW + x0, y0 * SW + x1, y1 * SW + x0, y1 * SW + x1, (x1 - x0) * (y1 - y0)};
 }
 
 // The sum over the rectangle from table k (pair k / 2, base = pair · table size in cells; every
@@ -338,6 +339,12 @@ __device__ __forceinline__ double sat_sum(const double2* __restrict__ S, int k, 
 // in translation plans those with bsat_of >= 0 from side B's. A rectangle plan reads canvas A's
 // tables (table k: the plan's k-th sum); a translation plan reads each image's own tables (table
 // k: plane k), both at an offset (SatView).
+//
+// Lag order: a thread takes the four lags (px, py), (px + nx/2, py), (px, py + ny/2) and
+// (px + nx/2, py + ny/2) one after the other (px < nx/2, py < ny/2). When the rectangles span half
+// the plane (images of equal size on a plane twice as large) those four lags read the same table
+// cells (a rectangle [d, h) and its complement [0, d) meet at row d, likewise for columns), so the
+// cells of one lag are still in the caches for the next three instead of being fetched four times.
 template <int PID>
 __global__ __launch_bounds__(256) void k_combine_rect(const float4* __restrict__ corr4, float* map, int64_t nn,
                                                       CombineArgs a, SatView sv, SatView svb, float* arg_part) {
@@ -347,10 +354,18 @@ __global__ __launch_bounds__(256) void k_combine_rect(const float4* __restrict__
   const uint32_t SP = sv.cells, SPB = svb.cells;
   SatIdx q = sat_idx(sv, 0), qb = q;
   if constexpr (TRANS) qb = sat_idx(svb, 0);
+  // a translation plan's sums come from the image tables (plane p in table p − 1); the moving
+  // image's weight plane is 1 on every pixel, so its sum over a rectangle is the area
   auto C = [&](int num, int64_t t) -> double {
-    if (P.sat_of[num] >= 0) return sat_sum(sv.sat, TRANS ? P.sat_plane[P.sat_of[num]] : P.sat_of[num], SP, q);
-    if constexpr (TRANS)
-      if (P.bsat_of[num] >= 0) return sat_sum(svb.sat, P.bsat_plane[P.bsat_of[num]], SPB, qb);
+    if constexpr (TRANS) {
+      if (P.sat_of[num] >= 0) {
+        const int pl = P.sat_plane[P.sat_of[num]];
+        return pl == 0 ? (double)q.area : sat_sum(sv.sat, pl - 1, SP, q);
+      }
+      if (P.bsat_of[num] >= 0) return sat_sum(svb.sat, P.bsat_plane[P.bsat_of[num]] - 1, SPB, qb);
+    } else {
+      if (P.sat_of[num] >= 0) return sat_sum(sv.sat, P.sat_of[num], SP, q);
+    }
     // the four correlations of a quad load together (k_rowifft_quads' layout)
     const int k = P.kept_of[num];
     const float4 v = corr4[(int64_t)(k >> 2) * nn + t];
@@ -361,7 +376,7 @@ __global__ __launch_bounds__(256) void k_combine_rect(const float4* __restrict__
   const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
   float bv = -3.4e38f;
   uint32_t bi = 0xffffffffu;
-  for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < nn; t += (int64_t)gridDim.x * 256) {
+  auto one = [&](int64_t t) {
     q = sat_idx(sv, t);
     if constexpr (TRANS) qb = sat_idx(svb, t);
     const float s = (float)lag_score<PID & 3>(C, t, floor_n, a, calib);
@@ -370,6 +385,15 @@ __global__ __launch_bounds__(256) void k_combine_rect(const float4* __restrict__
       bv = s;
       bi = (uint32_t)t;
     }
+  };
+  const int hx = sv.nx >> 1;
+  const int64_t nq = nn >> 2, hn = nn >> 1;
+  for (int64_t u = blockIdx.x * 256 + threadIdx.x; u < nq; u += (int64_t)gridDim.x * 256) {
+    const int64_t t = ((u / hx) << sv.lx) + (u & (hx - 1));
+    one(t);
+    one(t + hx);
+    one(t + hn);
+    one(t + hn + hx);
   }
   __shared__ float sv_[256];
   __shared__ uint32_t si_[256];
@@ -387,6 +411,156 @@ __global__ __launch_bounds__(256) void k_combine_rect(const float4* __restrict__
     arg_part[2 * blockIdx.x] = sv_[0];
     arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
   }
+}
+
+// k_combine for translation plans whose two images have the same size and sit in the canvas so
+// that every quad of lags (px, py), (px + nx/2, py), (px, py + ny/2), (px + nx/2, py + ny/2)
+// splits each image at one cell (R, C): per axis one lag row of the quad sums rows [R, hi) and
+// the other rows [0, R) (R = clamp(b + s·p), the axis' fixed rectangle edge moved by the lag),
+// likewise for columns (quad_view checks this on the host for every row and column: the moving
+// image takes the upper parts at lags >= 0, the fixed image the lower ones). Every rectangle sum
+// of the four lags then comes from four cells per table pair: S(R, C) and S(hi, C) (loaded once
+// per quad), S(R, hi) (one row per wave) and S(hi, hi) (one per table); the two uniform ones go
+// through LDS. That is 2 loads per pair and quad instead of 16. Each lag's sums are formed as
+// k_combine_rect forms them ((a − b) − (c − d) with the zero cells as 0), so the map is identical
+// bit for bit. Lag row ny/2 is taken at −ny/2 (both are empty: quad_view checks).
+__device__ __forceinline__ int quad_r(const QuadAxis& q, int p) { return min(max(q.b + q.s * p, q.c0), q.c1); }
+
+template <int PID>
+__global__ __launch_bounds__(256) void k_combine_quad(const float4* __restrict__ corr4, float* map, int64_t nn,
+                                                      int lx, CombineArgs a, QuadView va, QuadView vb, double n0,
+                                                      float* arg_part) {
+  constexpr CorrPlan P = kPlans[PID];
+  constexpr bool EXACT = PID & 1;
+  static_assert(P.n_bsat > 0, "translation plans only");
+  constexpr int NP = (kImageTableCount + 1) / 2;   // table pairs per image
+  const int hx = 1 << (lx - 1);
+  const int64_t nq = nn >> 2, hn = nn >> 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // per table pair (image A's NP, then image B's): S(hi, hi) − S(R, hi) and S(R, hi), per wave
+  __shared__ double2 uni[4][2 * NP][2];
+  double2 hh = make_double2(0.0, 0.0);
+  if (lane < NP) hh = va.sat[lane * va.cells + va.y.hi * va.stride + va.x.hi];
+  else if (lane < 2 * NP) hh = vb.sat[(lane - NP) * vb.cells + vb.y.hi * vb.stride + vb.x.hi];
+  const bool calib = EXACT && a.calib;
+  const double floor_n = calib ? (double)a.min_n : fmax((double)a.min_n, 0.08 * fmax(n0, 0.0));
+  float bv = -3.4e38f;
+  uint32_t bi = 0xffffffffu;
+  for (int64_t u = blockIdx.x * 256 + threadIdx.x; u < nq; u += (int64_t)gridDim.x * 256) {
+    const int py = (int)(u >> (lx - 1)), px = (int)(u & (hx - 1));   // py is the same across the wave
+    const int Ra = quad_r(va.y, py), Ca = quad_r(va.x, px), Rb = quad_r(vb.y, py), Cb = quad_r(vb.x, px);
+    double2 rc[2 * NP], hc[2 * NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      rc[p] = va.sat[p * va.cells + (uint32_t)Ra * va.stride + Ca];
+      hc[p] = va.sat[p * va.cells + (uint32_t)va.y.hi * va.stride + Ca];
+      rc[NP + p] = vb.sat[p * vb.cells + (uint32_t)Rb * vb.stride + Cb];
+      hc[NP + p] = vb.sat[p * vb.cells + (uint32_t)vb.y.hi * vb.stride + Cb];
+    }
+    if (lane < 2 * NP) {
+      const double2 rh = lane < NP ? va.sat[lane * va.cells + (uint32_t)Ra * va.stride + va.x.hi]
+                                   : vb.sat[(lane - NP) * vb.cells + (uint32_t)Rb * vb.stride + vb.x.hi];
+      uni[wv][lane][0] = make_double2(hh.x - rh.x, hh.y - rh.y);
+      uni[wv][lane][1] = rh;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t t = ((int64_t)py << lx) + px + ((j & 1) ? hx : 0) + ((j & 2) ? hn : 0);
+      // image A takes [R, hi) on an axis where the lag is >= 0 (j < 2 rows, even j columns), image
+      // B the complement
+      const bool uy = j < 2, ux = (j & 1) == 0;
+      // rectangle sum of table `comp` of pair `slot`, upper or lower part per axis
+      auto rect = [&](int slot, int comp, bool up_y, bool up_x) -> double {
+        const double vrc = comp ? rc[slot].y : rc[slot].x;
+        if (up_y && up_x) {
+          const double2 w = uni[wv][slot][0];
+          return (comp ? w.y : w.x) - ((comp ? hc[slot].y : hc[slot].x) - vrc);
+        }
+        if (up_y) return (comp ? hc[slot].y : hc[slot].x) - vrc;
+        if (up_x) {
+          const double2 w = uni[wv][slot][1];
+          return (comp ? w.y : w.x) - vrc;
+        }
+        return vrc;
+      };
+      auto C = [&](int num, int64_t tt) -> double {
+        if (P.sat_of[num] >= 0) {
+          const int pl = P.sat_plane[P.sat_of[num]];
+          if (pl == 0) {
+            const int ay = uy ? va.y.hi - Ra : Ra, ax = ux ? va.x.hi - Ca : Ca;
+            return (double)(ax * ay);
+          }
+          return rect((pl - 1) >> 1, (pl - 1) & 1, uy, ux);
+        }
+        if (P.bsat_of[num] >= 0) {
+          const int pl = P.bsat_plane[P.bsat_of[num]];
+          return rect(NP + ((pl - 1) >> 1), (pl - 1) & 1, !uy, !ux);
+        }
+        const int k = P.kept_of[num];
+        const float4 v = corr4[(int64_t)(k >> 2) * nn + tt];
+        return (double)((k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w);
+      };
+      const float sc = (float)lag_score<PID & 3, true>(C, t, floor_n, a, calib);
+      map[t] = sc;
+      if (better(sc, (uint32_t)t, bv, bi)) {
+        bv = sc;
+        bi = (uint32_t)t;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __shared__ float sv_[256];
+  __shared__ uint32_t si_[256];
+  sv_[threadIdx.x] = bv;
+  si_[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k && better(sv_[threadIdx.x + k], si_[threadIdx.x + k], sv_[threadIdx.x], si_[threadIdx.x])) {
+      sv_[threadIdx.x] = sv_[threadIdx.x + k];
+      si_[threadIdx.x] = si_[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    arg_part[2 * blockIdx.x] = sv_[0];
+    arg_part[2 * blockIdx.x + 1] = __uint_as_float(si_[0]);
+  }
+}
+
+// One axis of a SatView in the quad form, or false. Rows (n = ny) or columns (n = nx) p of the
+// first half pair with p − n/2; sat_idx clamps each edge b + s·d to [c0, c1].
+static bool quad_axis(int n, int b0, int b1, int s, int c0, int c1, QuadAxis* q) {
+  auto cl = [&](int v) { return std::min(std::max(v, c0), c1); };
+  if (n < 2 || c0 != 0) return false;
+  // lag ±n/2: both empty (the quad form takes row n/2 as −n/2, sat_idx as +n/2)
+  if (cl(b0 + s * (n / 2)) != cl(b1 + s * (n / 2)) || cl(b0 - s * (n / 2)) != cl(b1 - s * (n / 2))) return false;
+  for (int pat = 0; pat < 2; ++pat) {
+    const int hi = pat == 0 ? cl(b1) : cl(b1 - s * (n / 2));
+    bool ok = true;
+    for (int p = 0; p < n / 2 && ok; ++p) {
+      const int d = p, e = p - n / 2;
+      const int y0d = cl(b0 + s * d), y1d = cl(b1 + s * d), y0e = cl(b0 + s * e), y1e = cl(b1 + s * e);
+      ok = pat == 0 ? (y0d == y1e && y0e == 0 && y1d == hi) : (y1d == y0e && y0d == 0 && y1e == hi);
+    }
+    if (ok) {
+      *q = {pat == 0 ? b0 : b1, s, c0, c1, hi, pat};
+      return true;
+    }
+  }
+  return false;
+}
+
+// The moving image's view (sign +1) must take pattern 0 on both axes, the fixed image's (sign −1)
+// pattern 1, as k_combine_quad assumes.
+bool quad_view(const SatView& v, QuadView* q) {
+  if (v.nx < 128 || v.ny < 2) return false;   // a wave's 64 quads share one row
+  q->sat = v.sat;
+  q->stride = (uint32_t)v.stride;
+  q->cells = v.cells;
+  const int want = v.sign > 0 ? 0 : 1;
+  return quad_axis(v.ny, v.by0, v.by1, v.sign, v.cy0, v.cy1, &q->y) &&
+         quad_axis(v.nx, v.bx0, v.bx1, v.sign, v.cx0, v.cx1, &q->x) && q->y.patB == want && q->x.patB == want;
 }
 
 // ── roto-scale maps: per-lag score and finish ─────────────────────────────────────────────
@@ -506,6 +680,22 @@ int combine_blocks(int64_t nn) { return (int)std::min<int64_t>((nn + 255) / 256,
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,
                     hipStream_t st, const SatView* sv, const SatView* svb) {
   const int g = combine_blocks(nn);
+  // every score kernel writes g block partials (launch_argmax_finish reads combine_blocks(nn))
+  if (plan >= 8 && sv && svb && MapPath::trans == 1) {
+    QuadView qa{}, qb{};
+    if (quad_view(*sv, &qa) && quad_view(*svb, &qb) && sv->nx == svb->nx && sv->ny == svb->ny) {
+      const double n0 = (double)(std::min(std::max(sv->bx1, sv->cx0), sv->cx1) - std::min(std::max(sv->bx0, sv->cx0), sv->cx1)) *
+                        (double)(std::min(std::max(sv->by1, sv->cy0), sv->cy1) - std::min(std::max(sv->by0, sv->cy0), sv->cy1));
+      const float4* c4 = reinterpret_cast<const float4*>(corr);
+      switch (plan) {
+        case 8: k_combine_quad<8><<<g, 256, 0, st>>>(c4, map, nn, sv->lx, a, qa, qb, n0, arg_part); break;
+        case 9: k_combine_quad<9><<<g, 256, 0, st>>>(c4, map, nn, sv->lx, a, qa, qb, n0, arg_part); break;
+        case 10: k_combine_quad<10><<<g, 256, 0, st>>>(c4, map, nn, sv->lx, a, qa, qb, n0, arg_part); break;
+        default: k_combine_quad<11><<<g, 256, 0, st>>>(c4, map, nn, sv->lx, a, qa, qb, n0, arg_part); break;
+      }
+      return;
+    }
+  }
   if (plan >= 4 && !sv) throw std::runtime_error("rectangle plan without tables");
   if (plan >= 8 && !svb) throw std::runtime_error("translation plan without the fixed side's tables");
   const SatView b = svb ? *svb : SatView{};

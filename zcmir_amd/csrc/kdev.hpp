@@ -152,8 +152,12 @@ __device__ __forceinline__ double f_to_z(double f, double d1, double d2) {
 }
 
 // In-place Cholesky of the lower triangle of g (NF x NF); rd[i] = 1 / L_ii. False if not PD.
-template <int NF>
+// FLAT: without branches — a pivot that is not positive marks the result false and is replaced by
+// 1 so that the remaining steps stay finite (their values are then discarded); for a positive
+// definite g every step is the plain one, so both forms give the same factor.
+template <int NF, bool FLAT = false>
 __device__ __forceinline__ bool chol(double (&g)[4][4], double (&rd)[4]) {
+  bool ok = true;
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
 #pragma unroll
@@ -162,7 +166,9 @@ __device__ __forceinline__ bool chol(double (&g)[4][4], double (&rd)[4]) {
 #pragma unroll
       for (int k = 0; k < j; ++k) s -= g[i][k] * g[j][k];
       if (i == j) {
-        if (!(s > 0.0)) return false;
+        if (!FLAT && !(s > 0.0)) return false;
+        ok = ok && s > 0.0;
+        s = s > 0.0 ? s : 1.0;
         const double r = rsqrt(s);   // 1 / L_ii without a division
         g[i][i] = s * r;
         rd[i] = r;
@@ -171,12 +177,12 @@ __device__ __forceinline__ bool chol(double (&g)[4][4], double (&rd)[4]) {
       }
     }
   }
-  return true;
+  return ok;
 }
 
 // The per-lag score (smi.wgsl:825-931 combine_cell) from the correlation planes read through
 // C(number, lag): 0 below the overlap floor; FP64 throughout.
-template <int PID, typename CF>
+template <int PID, bool FLAT = false, typename CF>
 __device__ __forceinline__ double lag_score(CF&& C, int64_t t, double floor_n, const CombineArgs& a, bool calib) {
   constexpr bool EXACT = PID & 1;
   constexpr int NF = (PID & 2) ? 3 : 4;
@@ -221,7 +227,14 @@ __device__ __forceinline__ double lag_score(CF&& C, int64_t t, double floor_n, c
       gb[i][i] += rb;
     }
     double ra_[4], rb_[4];
-    if (chol<NF>(ga, ra_) && chol<NF>(gb, rb_)) {
+    bool pd;
+    if constexpr (FLAT) {
+      const bool pda = chol<NF, true>(ga, ra_), pdb = chol<NF, true>(gb, rb_);
+      pd = pda && pdb;
+    } else {
+      pd = chol<NF>(ga, ra_) && chol<NF>(gb, rb_);
+    }
+    if (pd) {
       // X = L_A⁻¹ C (columns), then Y = X L_B⁻ᵀ (rows), acc = ‖Y‖²
       double x[4][4];
 #pragma unroll

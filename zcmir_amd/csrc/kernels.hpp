@@ -66,7 +66,8 @@ bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, i
 bool launch_rowifft_quads(int plan, const float2* work, float4* corr4, int nx, int ny, const float2* tw_x,
                           hipStream_t st);
 // Maps at integer-translation poses (full resolution): 1 (default) the translation plans, 0 the
-// rectangle plans every other full-resolution pose takes; process-wide (zc_map_path)
+// rectangle plans every other full-resolution pose takes, 2 the translation plans with every lag
+// reading its own table cells (no quad form); process-wide (zc_map_path)
 struct MapPath {
   static inline std::atomic<int> trans{1};
 };
@@ -99,6 +100,17 @@ struct SatView {
   int cx0, cx1, cy0, cy1, stride;
   uint32_t cells;
 };
+// The quad form of a translation plan's table view (k_combine_quad); quad_view fills it when the
+// view allows it.
+struct QuadAxis {
+  int b, s, c0, c1, hi, patB;
+};
+struct QuadView {
+  const double2* sat;
+  uint32_t stride, cells;
+  QuadAxis y, x;
+};
+bool quad_view(const SatView& v, QuadView* q);
 // sv: side A's tables (rectangle plans); svb: side B's (translation plans); rectangle plans read
 // corr in launch_rowifft_quads' layout, the others one plane per correlation
 void launch_combine(int plan, const float* corr, float* map, int64_t nn, const CombineArgs& a, float* arg_part,

@@ -151,14 +151,17 @@ struct SatSet {
 };
 
 // sid < 16: side A's tables of plan sid; 16 + plan: side B's; kImageTables: an image's own tables,
-// table k of plane k for all 15 planes (translation plans read both sides' sums from these).
+// table k of plane k + 1 for the 14 planes other than the weight (translation plans read both
+// sides' sums from these; an image's weight plane is 1 on every pixel, so its rectangle sums are
+// the rectangles' areas and need no table).
 inline constexpr int kImageTables = 32;
+inline constexpr int kImageTableCount = 14;
 
 constexpr SatSet sat_set(int sid) {
   SatSet t{};
   if (sid == kImageTables) {
-    t.n = 15;
-    for (int k = 0; k < 15; ++k) t.plane[k] = k;
+    t.n = kImageTableCount;
+    for (int k = 0; k < kImageTableCount; ++k) t.plane[k] = k + 1;
     return t;
   }
   const CorrPlan& p = kPlans[sid & 15];
