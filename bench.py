@@ -86,8 +86,10 @@ def _plan(three, rect, trans=False):
 
 def bench_refine(a, np, torch, zcmir_amd, synth):
     """The refine loop (zc_climb: shift-map hops, Gauss–Newton blocks, roto-scale hops) on the
-    1024² map pair from a start pose 6 px and 2° off the truth: wall time per climb against the
-    GPU time of its launches (HIP events), i.e. what the host-driven loop costs over its kernels."""
+    1024² map pair from a start pose 6 px and 2° off the truth. Wall time per climb with its
+    decisions on the device (climb_graph: one captured graph per iteration, the default) and with
+    the host-driven loop, against the GPU time of the climb's launches (HIP events around every
+    launch in one more, profiled climb): host_share = 1 − GPU time / wall time."""
     import math
     import time
 
@@ -98,29 +100,41 @@ def bench_refine(a, np, torch, zcmir_amd, synth):
     H0 = C @ R @ np.linalg.inv(C) @ Ht @ np.array([[1, 0, 6.0], [0, 1, -4.0], [0, 0, 1]])
     reg = zcmir_amd.Registrar(half=False, symmetric=True, hop=True, hop_fm=True, g_iters=25)
     reg.set_images(mv, fx, preprocess=False)
-    reg.pose = H0
-    c = reg.climb()   # warm-up: tilings, buffers
-    walls, gpu = [], []
-    for _ in range(a.refine_runs):
+
+    def walls_of(graph):
+        reg.configure(climb_graph=graph)
         reg.pose = H0
-        torch.cuda.synchronize()
-        reg.profile(True)
-        t0 = time.perf_counter()
-        c = reg.climb()
-        torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
-        gpu.append(sum(tot for _, tot, _ in reg.profile_report().values()) * 1e-3)
-        reg.profile(False)
+        reg.climb()   # warm-up: tilings, buffers, the captured graph
+        walls = []
+        for _ in range(a.refine_runs):
+            reg.pose = H0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            c = reg.climb()
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+        return float(np.median(walls)), c
+
+    w_host, c_host = walls_of(False)
+    w, c = walls_of(True)
+    reg.pose = H0
+    reg.profile(True)
+    reg.climb()
+    rep = reg.profile_report()
+    g = sum(tot for _, tot, _ in rep.values()) * 1e-3
+    reg.profile(False)
+    top = sorted(rep.items(), key=lambda kv: -kv[1][1])[:10]
     corners = np.array([[0.5, 0.5, 1], [1024.5, 0.5, 1], [1024.5, 1024.5, 1], [0.5, 1024.5, 1]]).T
     p, q = c.H @ corners, Ht @ corners
     err = float(np.max(np.hypot(*(p[:2] / p[2] - q[:2] / q[2]))))
     reg.close()
-    w, g = float(np.median(walls)), float(np.median(gpu))
     return {"metric": "refine (climb) wall time", "unit": "ms", "workload": "1024x1024 map pair, start 6 px and "
             "2 deg off the truth, symmetric SMI, shift-map and roto-scale hops, Gauss-Newton blocks",
-            "ms_per_climb": round(1e3 * w, 2), "gpu_ms_per_climb": round(1e3 * g, 2),
-            "host_share": round(1 - g / w, 3), "iterations": int(c.iterations), "corner_err_px": round(err, 3),
-            "runs": a.refine_runs}
+            "ms_per_climb": round(1e3 * w, 2), "ms_per_climb_host_loop": round(1e3 * w_host, 2),
+            "gpu_ms_per_climb": round(1e3 * g, 2), "host_share": round(1 - g / w, 3),
+            "host_share_host_loop": round(1 - g / w_host, 3), "iterations": int(c.iterations),
+            "iterations_host_loop": int(c_host.iterations), "corner_err_px": round(err, 3), "runs": a.refine_runs,
+            "stages_ms_profiled": {k: [v[0], round(v[1], 3)] for k, v in top}}
 
 
 def bench_map2048(a, np, torch, zcmir_amd, synth):

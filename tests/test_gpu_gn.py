@@ -68,6 +68,40 @@ def test_climb_matches_oracle(reg_factory, offset, hop, hop_fm):
     assert np.allclose(reg.pose, c.H)
 
 
+@pytest.mark.parametrize("offset,hop,hop_fm,symmetric", [((1.5, -1.0, 0.0, 1.0), False, False, True),
+                                                         ((6.5, -4.0, 0.0, 1.0), True, False, True),
+                                                         ((2.0, 1.0, 7.0, 1.06), True, True, True),
+                                                         ((2.0, 1.0, 7.0, 1.06), True, True, False)])
+def test_climb_on_device_matches_host_loop(reg_factory, offset, hop, hop_fm, symmetric):
+    """The climb with its decisions on the device (climb_graph, the default: one captured graph
+    per iteration) takes the host-driven loop's path: the same iteration count and end pose; a
+    second climb from the same start replays the cached graph from its first iteration and ends
+    at the same pose bit for bit."""
+    import math
+
+    mv, fx, Ht = synth.shift_pair((96, 96), (5, -3), seed=2)
+    tx, ty, deg, sc = offset
+    a = math.radians(deg)
+    C = np.array([[1, 0, 48.5], [0, 1, 48.5], [0, 0, 1.0]])
+    R = np.array([[sc * math.cos(a), -sc * math.sin(a), 0], [sc * math.sin(a), sc * math.cos(a), 0], [0, 0, 1]])
+    H0 = C @ R @ np.linalg.inv(C) @ Ht @ np.array([[1, 0, tx], [0, 1, ty], [0, 0, 1]])
+    out = {}
+    for graph in (False, True):
+        reg = reg_factory(half=False, g_iters=25, hop=hop, hop_fm=hop_fm, symmetric=symmetric, climb_graph=graph)
+        reg.set_images(mv, fx, preprocess=False)
+        reg.pose = H0
+        out[graph] = reg.climb()
+        if graph:
+            reg.pose = H0
+            again = reg.climb()
+            assert again.iterations == out[graph].iterations
+            assert np.array_equal(again.H, out[graph].H) and again.score == out[graph].score
+    d, h = out[True], out[False]
+    assert d.iterations == h.iterations and d.moved == h.moved, (d.iterations, h.iterations)
+    assert _corner_err(d.H, h.H, 96, 96) < 1e-3
+    assert abs(d.score - h.score) <= 1e-6 * abs(h.score)
+
+
 def test_search_exact_quarter_turn(reg_factory):
     n = 96
     _, fixed, _ = synth.shift_pair((n, n), (0, 0), seed=5)

@@ -93,7 +93,8 @@ Engine::Engine(int device) : dev(device) {
   hipDeviceProp_t p;
   ZC_HIP(hipGetDeviceProperties(&p, dev));
   device_name = std::string(p.name) + " (" + p.gcnArchName + ")";
-  ZC_HIP(hipHostMalloc((void**)&res_host, 64, hipHostMallocDefault));
+  // [0, 64): map and roto-scale peaks; [64, 256): the device climb's state
+  ZC_HIP(hipHostMalloc((void**)&res_host, 256, hipHostMallocDefault));
 }
 
 Engine::~Engine() {
@@ -106,10 +107,12 @@ Engine::~Engine() {
                   &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
                   &gn_sums, &sat_band, &sat, &rs_fwd, &rs_inv, &rs_out, &rs_tab, &pre_buf, &pre_hist, &pre_cdf,
                   &pre_part, &pre_rgba, &map_out, &gd_pairs, &gd_state, &gd_gjobs, &gd_gsums, &gd_tjobs, &gd_tsums,
-                  &gd_dir, &gd_step, &gd_try_H, &gd_try_ok, &gd_try_jobs, &gd_try_sums, &gd_try_score, &gd_part, &gd_ctr})
+                  &gd_dir, &gd_step, &gd_try_H, &gd_try_ok, &gd_try_jobs, &gd_try_sums, &gd_try_score, &gd_part, &gd_ctr,
+                  &cl_state, &cells_buf})
     b->release();
   sweep_gc.reset();
   gn_gc.reset();
+  climb_gc.reset();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -534,7 +537,7 @@ MapOut Engine::shift_map(const Mat3& Hp, bool exact, bool calib, uint32_t cap, d
 }
 
 MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, bool calib, uint32_t cap,
-                               double min_overlap, bool wait, const Frame* frame) {
+                               double min_overlap, bool wait, const Frame* frame, const double* dev_hi) {
   if (A.mode != B.mode) throw std::runtime_error("FeatureModeMismatch: set both images with the same smi_features");
   const bool three = A.mode == "three";
   MapOut m{};
@@ -567,7 +570,7 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   const bool int_shift = Hc[0] == 1.0 && Hc[1] == 0.0 && Hc[3] == 0.0 && Hc[4] == 1.0 && Hc[6] == 0.0 &&
                          Hc[7] == 0.0 && Hc[8] == 1.0 && Hc[2] == std::floor(Hc[2]) && Hc[5] == std::floor(Hc[5]) &&
                          std::fabs(Hc[2]) < 1e6 && std::fabs(Hc[5]) < 1e6;
-  const bool trans = rect && int_shift && MapPath::trans != 0;
+  const bool trans = rect && int_shift && MapPath::trans != 0 && !dev_hi;
   const int plan = (exact ? 1 : 0) + (three ? 2 : 0) + (rect ? 4 : 0) + (trans ? 4 : 0);
   const CorrPlan& pi = kPlans[plan];
   const int64_t N = (int64_t)nx * ny;
@@ -599,12 +602,15 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   double Hi[9];
   warp_inverse(Hc, Hi);
   const float2* y_tw = twiddles(ny);
-  LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
+  if (dev_hi)
+    LAUNCH("warp_stack", launch_warp_stack_dev(A.stack.as<float>(), A.w, A.h, dev_hi, canvas_a.as<float>(), W, Hh, st));
+  else
+    LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
   LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
   bool ok = true;
   // A rectangle plan's tables depend only on canvas A: unless launches are profiled (one event
   // chain on one stream), they are built on the second stream while the FFT stages run.
-  const bool fork = rect && !trans && !prof.on;
+  const bool fork = rect && !trans && !prof.on && !dev_hi;
   hipStream_t sst = fork ? st2 : st;
   if (fork) {
     ZC_HIP(hipEventRecord(ev_fork, st));
@@ -638,8 +644,13 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
            (uint32_t)((B.w + 1) * (B.h + 1))};
   }
   if (fork) ZC_HIP(hipEventRecord(ev_join, st2));
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st));
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st));
+  float* cells = nullptr;
+  if (!rect) {
+    cells_buf.ensure((size_t)cw * ch * 5 * 4);
+    cells = cells_buf.as<float>();
+  }
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st, cells));
+  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st, cells));
   LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   LAUNCH("cmul", launch_cmul(plan, spa, spb, work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));
@@ -660,10 +671,11 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   if (fork) ZC_HIP(hipStreamWaitEvent(st, ev_join, 0));
   LAUNCH("combine", launch_combine(plan, corr.as<float>(), map.as<float>(), N, ca, arg_part.as<float>(), st, rect ? &sv : nullptr, trans ? &svb : nullptr));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, map.as<float>(), res.as<float>(), st));
+  ++counts[1];
+  if (dev_hi) return m;
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
   prof.break_chain();
   ++counts[2];
-  ++counts[1];
   if (wait) {
     sync();
     std::memcpy(m.res, res_host, 12);

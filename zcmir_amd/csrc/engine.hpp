@@ -41,6 +41,9 @@ namespace zc {
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
+  // counts every allocation in the process: a captured graph whose key holds it is not replayed
+  // after any buffer moved
+  static inline std::atomic<uint64_t> gen{0};
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
     if (p) ZC_HIP(hipFree(p));
@@ -48,6 +51,7 @@ struct DBuf {
     cap = 0;
     ZC_HIP(hipMalloc(&p, bytes));
     cap = bytes;
+    ++gen;
   }
   void release() {
     if (p) (void)hipFree(p);
@@ -153,8 +157,8 @@ struct Side {
   // on first use (a cache of feat: const users may fill it)
   mutable DBuf il;
   mutable bool il_ok = false;
-  // the image's own summed-area tables of its 15 stack planes (pairs), made on first use by a
-  // translation-plan map
+  // the image's own summed-area tables of its 14 stack planes other than the weight (pairs), made
+  // on first use by a translation-plan map
   mutable DBuf tab;
   mutable bool tab_ok = false;
   float mean = 0, spread = 1;
@@ -283,8 +287,11 @@ class Engine {
   MapOut shift_map(const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap, bool wait = true);
   // frame: the canvas (default: H's own, reg_frame); the climb keeps one frame for a batch of
   // iterations (gclimb.zig:384-387)
+  // dev_hi: the canvas → moving inverse pose in device memory (9 doubles, a pose device code
+  // chose): the moving stack is warped from it, the plan is the one of a general pose (no
+  // translation plan), and nothing is copied to the host (the peak stays in `res`); wait = false.
   MapOut shift_map_sides(Side& A, Side& B, const Mat3& H, bool exact, bool calib, uint32_t cap, double min_overlap,
-                         bool wait = true, const Frame* frame = nullptr);
+                         bool wait = true, const Frame* frame = nullptr, const double* dev_hi = nullptr);
   float* res_host = nullptr;   // pinned
   void fill_shift_result(const MapOut& m, ShiftResult* out) const;
   void read_map(float* out, size_t len, const MapOut& m);
@@ -298,13 +305,19 @@ class Engine {
   };
   // One direction: side A warped by H onto B's canvas, both on the log-polar grid about c
   // (1-based fixed pixels); r0 < 0 / r1 <= 0: from the canvas. The n x n map into dst.
+  // dev_hi: as for shift_map_sides
   RsPlan rs_map_dir(Side& A, Side& B, const Mat3& H, double c0, double c1, double r0, double r1, bool exact,
-                    bool calib, uint32_t cap, DBuf& dst, const Frame* frame = nullptr);
-  // Finish (symmetric: average with rs_inv) into rs_out and its peak into res_host.
-  void rs_finish(const RsPlan& p, bool symmetric, bool mirror);
+                    bool calib, uint32_t cap, DBuf& dst, const Frame* frame = nullptr, const double* dev_hi = nullptr);
+  // Finish (symmetric: average with rs_inv) into rs_out and its peak into res (and res_host unless
+  // to_host is false).
+  void rs_finish(const RsPlan& p, bool symmetric, bool mirror, bool to_host = true);
   // The climb from the engine's pose (gclimb.zig:313-489): shift-map hops, roto-scale hops and
   // Gauss–Newton blocks per iteration; the engine's pose becomes the result.
   ClimbResult climb(uint32_t n_iters, bool hop, bool hop_fm, bool homography, bool exact, bool symmetric);
+  // The same climb with its decisions on the device (climb_graph): one captured graph per iteration
+  // (map hop, Gauss–Newton blocks, roto-scale hop, stall and stop rule over device state); the host
+  // reads the state once per iteration and sets each batch's frame.
+  ClimbResult climb_device(uint32_t n_iters, bool hop, bool hop_fm, bool homography, bool exact, bool symmetric);
 
   void moments(int a_side, int b_side, const Mat3& H, double out[45]);
   // SMI from 45 sums (moments.zig:91-134), FP64
@@ -322,6 +335,7 @@ class Engine {
   };
   GraphCache sweep_gc;
   GraphCache gn_gc;   // one Gauss-Newton iteration of the device batch
+  GraphCache climb_gc;   // one iteration of the device climb
   // Issues `issue` (kernel launches only on st: no copies, fills, syncs or allocations) through
   // gc's graph: replayed when the key matches, else captured, instantiated and launched. Runs
   // `issue` eagerly when `eager` (profiling needs per-launch events).
@@ -380,6 +394,7 @@ class Engine {
 
   DBuf canvas_a, canvas_b, spec_a, spec_b, work, corr, map, arg_part, res, parts, sums, tmp_u32, tmp_i64;
   DBuf map_out;   // a map laid out on the reference plan's grid (read_map)
+  DBuf cells_buf;   // a resampled map's cell values (launch_pack_rowfft)
   DBuf sat_band, sat;   // rectangle plans: band sums and summed-area tables
   DBuf rs_fwd, rs_inv, rs_out, rs_tab;   // roto-scale maps: both directions, finished, log-polar tables
   DBuf pre_buf, pre_hist, pre_cdf, pre_part, pre_rgba;   // preprocessing scratch
@@ -388,6 +403,7 @@ class Engine {
   // the device Gauss-Newton batch (GnBuffers)
   DBuf gd_pairs, gd_state, gd_gjobs, gd_gsums, gd_tjobs, gd_tsums, gd_dir, gd_step, gd_try_H, gd_try_ok, gd_try_jobs,
       gd_try_sums, gd_try_score, gd_part, gd_ctr;
+  DBuf cl_state;   // the device climb: ClimbDev, then the warp's inverse pose (9 doubles)
   std::map<int, DBuf> tw;
   const float2* twiddles(int L);
   // Column-FFT tiling for column length L (launch_colfft's `lines`): zc_fft_tune's value when set,

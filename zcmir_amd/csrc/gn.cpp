@@ -188,7 +188,7 @@ void Engine::moments_jobs(const std::vector<DirJob>& d, std::vector<std::array<d
     seen.insert(&b);
   }
   for (const Side* s : seen) feat_bytes += 16.0 * s->w * s->h;
-  const int blocks = jobs_blocks(px);
+  const int blocks = jobs_blocks(px, (int)nj);
   gn_jobs.ensure(nj * sizeof(PoseJob));
   gn_part.ensure(nj * blocks * 45 * sizeof(double));
   gn_sums.ensure(nj * 45 * sizeof(double));
@@ -251,7 +251,7 @@ void Engine::tangent_jobs(const std::vector<DirJob>& d, const std::vector<Coefs>
     seen.insert(&b);
   }
   for (const Side* s : seen) feat_bytes += 16.0 * s->w * s->h;
-  const int blocks = jobs_blocks(px);
+  const int blocks = jobs_blocks(px, (int)nj);
   gn_jobs.ensure(nj * sizeof(TangentJob));
   gn_part.ensure(nj * blocks * 44 * sizeof(double));
   gn_sums.ensure(nj * 44 * sizeof(double));
@@ -497,7 +497,7 @@ std::vector<GnOut> Engine::gn_batch_device(const std::vector<PairSides>& pr, con
   for (size_t k = 0; k < amps.size(); ++k) ga.amps[k] = amps[k];
   ga.n_try = (int)amps.size() + 5;
   const int nd = (symmetric ? 2 : 1) * (int)n, T = ga.n_try;
-  const int blocks = jobs_blocks(px);
+  const int blocks = jobs_blocks(px, nd), blocks_try = jobs_blocks(px, nd * T);
   gd_pairs.ensure(n * sizeof(GnPair));
   gd_state.ensure(n * sizeof(GnState));
   gd_gjobs.ensure(nd * sizeof(PoseJob));
@@ -511,7 +511,7 @@ std::vector<GnOut> Engine::gn_batch_device(const std::vector<PairSides>& pr, con
   gd_try_jobs.ensure((size_t)nd * T * sizeof(PoseJob));
   gd_try_sums.ensure((size_t)nd * T * 45 * 8);
   gd_try_score.ensure((size_t)nd * T * 8);
-  gd_part.ensure((size_t)nd * T * blocks * 45 * 8);
+  gd_part.ensure((size_t)std::max(nd * blocks, nd * T * blocks_try) * 45 * 8);
   gd_ctr.ensure(2 * 4);
   ZC_HIP(hipMemcpyAsync(gd_pairs.p, pairs.data(), n * sizeof(GnPair), hipMemcpyHostToDevice, st));
   ZC_HIP(hipMemcpyAsync(gd_state.p, state.data(), n * sizeof(GnState), hipMemcpyHostToDevice, st));
@@ -530,9 +530,10 @@ std::vector<GnOut> Engine::gn_batch_device(const std::vector<PairSides>& pr, con
   put(gb);
   put(ga);
   put(blocks);
+  put(blocks_try);
   for (uint32_t it = 0; it < iters && !cancel; ++it)
     run_captured(gn_gc, std::vector<uint8_t>(key), prof.on,
-                 [&] { LAUNCH("gn_iteration", launch_gn_iteration(gb, ga, blocks, blocks, st)); });
+                 [&] { LAUNCH("gn_iteration", launch_gn_iteration(gb, ga, blocks, blocks_try, st)); });
   int done = 0;
   ZC_HIP(hipMemcpyAsync(state.data(), gd_state.p, n * sizeof(GnState), hipMemcpyDeviceToHost, st));
   ZC_HIP(hipMemcpyAsync(&done, gd_ctr.as<int>() + 1, 4, hipMemcpyDeviceToHost, st));

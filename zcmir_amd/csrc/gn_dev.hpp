@@ -189,6 +189,52 @@ __device__ __forceinline__ bool solve_linear(const double* A, const double* b, i
   return true;
 }
 
+// solve_linear by a workgroup of at least n · n threads (t: the thread's index) on M (n × n) and
+// x (n) in LDS, in place: the same operations on every element as the one-thread form above, with
+// the pivot row found by thread 0 and the rows' factors taken before any element changes; the
+// result is uniform over the workgroup (piv, f: n + 1 ints and n doubles of LDS scratch).
+__device__ __forceinline__ bool solve_linear_wg(double* M, double* x, int n, int t, int* piv, double* f) {
+  const int r = t / n, j = t - (t / n) * n;
+  for (int i = 0; i < n; ++i) {
+    if (t == 0) {
+      int p = i;
+      for (int q = i + 1; q < n; ++q)
+        if (fabs(M[q * n + i]) > fabs(M[p * n + i])) p = q;
+      *piv = p;
+    }
+    __syncthreads();
+    const int p = *piv;
+    if (p != i) {
+      if (t < n) {
+        const double v = M[i * n + t];
+        M[i * n + t] = M[p * n + t];
+        M[p * n + t] = v;
+      }
+      if (t == 0) {
+        const double v = x[i];
+        x[i] = x[p];
+        x[p] = v;
+      }
+    }
+    __syncthreads();
+    const double d = M[i * n + i];
+    if (!isfinite(d) || fabs(d) < 1e-300) return false;
+    if (t < n) f[t] = t == i ? 0.0 : M[t * n + i] / d;
+    __syncthreads();
+    if (r < n && r != i && j >= i && f[r] != 0) M[r * n + j] -= f[r] * M[i * n + j];
+    if (t < n && t != i && f[t] != 0) x[t] -= f[t] * x[i];
+    __syncthreads();
+  }
+  if (t < n) {
+    x[t] /= M[t * n + t];
+    piv[1 + t] = isfinite(x[t]) ? 1 : 0;
+  }
+  __syncthreads();
+  bool ok = true;
+  for (int q = 0; q < n; ++q) ok = ok && piv[1 + q];
+  return ok;
+}
+
 __device__ __forceinline__ bool beats(double nw, double old) { return nw > old + fmax(1e-6 * fabs(old), 1e-12); }
 
 __device__ __forceinline__ int tri(int i, int j) {

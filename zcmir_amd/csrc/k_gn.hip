@@ -169,6 +169,16 @@ __global__ __launch_bounds__(64) void k_gn_step(const GnPair* __restrict__ pairs
     if (t < nk) g[t] = pulled;
     __syncthreads();
   }
+  // (A + 1e-4 max(A_ii, 1e-12)) δ = ½ g by the whole workgroup
+  __shared__ double A[64], half[8], fac[8];
+  __shared__ int piv[9];
+  {
+    const int x = t / nk, c = t - (t / nk) * nk;
+    if (x < nk) A[x * nk + c] = h[x][c] + (x == c ? 1e-4 * fmax(h[x][c], 1e-12) : 0.0);
+    if (t < nk) half[t] = 0.5 * g[t];
+  }
+  __syncthreads();
+  const bool solved = gd::solve_linear_wg(A, half, nk, t, piv, fac);
   if (t != 0) return;
   s.score = score;
   // step_out[i]: u[8], step[8], |step|, flags (bit 0: ascent tries, bit 1: step tries)
@@ -184,17 +194,14 @@ __global__ __launch_bounds__(64) void k_gn_step(const GnPair* __restrict__ pairs
   double nd = 0;
   for (int k = 0; k < nk; ++k) nd += u[k] * u[k];
   int flags = (a.n_amps > 0 && sqrt(nd) > 1e-12) ? 1 : 0;
-  double A[64], half[8], step[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int x = 0; x < nk; ++x)
-    for (int c = 0; c < nk; ++c) A[x * nk + c] = h[x][c];
-  for (int q = 0; q < nk; ++q) {
-    A[q * nk + q] += 1e-4 * fmax(A[q * nk + q], 1e-12);
-    half[q] = 0.5 * g[q];
-  }
+  double step[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double sn = 0;
-  if (gd::solve_linear(A, half, nk, step)) {
+  if (solved) {
     flags |= 2;
-    for (int q = 0; q < nk; ++q) sn += step[q] * step[q];
+    for (int q = 0; q < nk; ++q) {
+      step[q] = half[q];
+      sn += step[q] * step[q];
+    }
     sn = sqrt(sn);
   }
   for (int k = 0; k < 8; ++k) {
@@ -308,5 +315,172 @@ void launch_gn_iteration(const GnBuffers& b, const GnArgs& a, int blocks_grad, i
   k_gn_select<<<g1, 64, 0, st>>>(b.state, a, b.try_H, b.try_ok, b.try_score, b.moved_flag);
   k_gn_tick<<<1, 64, 0, st>>>(b.moved_flag, b.iters_done);
 }
+
+// ── the climb ────────────────────────────────────────────────────────────────────────────
+// One thread each: the climb's decisions on pose 0 of a one-pose batch, gated by c->done so that
+// a replayed iteration after the end changes nothing.
+
+__global__ void k_climb_begin(ClimbDev* c, GnState* state, ClimbArgs a, double* hi, int reset_moved) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->done) return;
+  if (reset_moved) c->moved = 0;
+  // canvas pose T(−ox, −oy) · H and its inverse (geometry.hpp canvas_h, warp_inverse)
+  const double T[9] = {1, 0, -(double)a.ox, 0, 1, -(double)a.oy, 0, 0, 1};
+  double Hc[9];
+  gd::mul3(T, state[0].H, Hc);
+  const double det = Hc[0] * (Hc[4] * Hc[8] - Hc[5] * Hc[7]) - Hc[1] * (Hc[3] * Hc[8] - Hc[5] * Hc[6]) +
+                     Hc[2] * (Hc[3] * Hc[7] - Hc[4] * Hc[6]);
+  if (fabs(det) >= 1e-12) {
+    hi[0] = (Hc[4] * Hc[8] - Hc[5] * Hc[7]) / det;
+    hi[1] = (Hc[2] * Hc[7] - Hc[1] * Hc[8]) / det;
+    hi[2] = (Hc[1] * Hc[5] - Hc[2] * Hc[4]) / det;
+    hi[3] = (Hc[5] * Hc[6] - Hc[3] * Hc[8]) / det;
+    hi[4] = (Hc[0] * Hc[8] - Hc[2] * Hc[6]) / det;
+    hi[5] = (Hc[2] * Hc[3] - Hc[0] * Hc[5]) / det;
+    hi[6] = (Hc[3] * Hc[7] - Hc[4] * Hc[6]) / det;
+    hi[7] = (Hc[1] * Hc[6] - Hc[0] * Hc[7]) / det;
+    hi[8] = (Hc[0] * Hc[4] - Hc[1] * Hc[3]) / det;
+  } else {
+    for (int k = 0; k < 9; ++k) hi[k] = 0;
+  }
+}
+
+// hop_cand: T(−dx, −dy) · H from the shift map's peak; rs_cand: simAbout(c, dθ, dσ) · H from the
+// roto-scale map's (none within 2e-4 of no change)
+__global__ __launch_bounds__(64) void k_climb_hop(const GnPair* __restrict__ pairs, GnState* state, GnArgs ga,
+                                                  ClimbDev* c, ClimbArgs a, const float* __restrict__ res, int rs,
+                                                  double* try_H, int* try_ok, PoseJob* jobs) {
+  const int T = ga.n_try, k = threadIdx.x;
+  if (k >= T) return;
+  PoseJob* jb = ga.symmetric ? &jobs[T + k] : nullptr;
+  idle_job(jobs[k]);
+  if (jb) idle_job(*jb);
+  try_ok[k] = 0;
+  if (k != 0) return;   // slot 0 (this thread's own, idled just above) may hold the hop
+  GnState& s = state[0];
+  s.active = c->done ? 0 : 1;
+  if (c->done) return;
+  const uint32_t idx = __float_as_uint(res[1]);
+  double Ht[9];
+  if (!rs) {
+    const uint64_t nn = (uint64_t)a.map_nx * a.map_ny;
+    if (idx >= nn) return;
+    const int64_t py = idx / a.map_nx, px = idx % a.map_nx;
+    const int64_t lx = px <= a.map_nx / 2 ? px : px - (int64_t)a.map_nx;
+    const int64_t ly = py <= a.map_ny / 2 ? py : py - (int64_t)a.map_ny;
+    const double dx = (double)lx * a.cell_x, dy = (double)ly * a.cell_y;
+    const double Tm[9] = {1, 0, -dx, 0, 1, -dy, 0, 0, 1};
+    gd::mul3(Tm, s.H, Ht);
+  } else {
+    const int n = a.rs_n;
+    if (idx >= (uint64_t)n * n) return;
+    const int64_t py = idx / n, px = idx % n;
+    const int64_t lx = px <= n / 2 ? px : px - n;
+    const int64_t ly = py <= n / 2 ? py : py - n;
+    const double dth = -((double)lx / (double)max(a.rs_nth, 1)) * 2.0 * M_PI;
+    const double dsg = -(double)ly * a.rs_dlam;
+    if (fabs(dth) < 2e-4 && fabs(dsg) < 2e-4) return;
+    // simAbout (search.hpp sim_about): X ↦ c + e^dσ R(dθ) (X − c)
+    const double sc = exp(dsg);
+    const double m00 = sc * cos(dth), m01 = -sc * sin(dth), m10 = sc * sin(dth), m11 = sc * cos(dth);
+    const double S[9] = {m00, m01, a.rs_c0 - m00 * a.rs_c0 - m01 * a.rs_c1,
+                         m10, m11, a.rs_c1 - m10 * a.rs_c0 - m11 * a.rs_c1, 0, 0, 1};
+    gd::mul3(S, s.H, Ht);
+    gd::project_group(Ht, a.homography != 0);
+  }
+  for (int q = 0; q < 9; ++q) try_H[q] = Ht[q];
+  try_ok[0] = 1;
+  pose_jobs(pairs[0], Ht, ga.symmetric, jobs[0], jb);
+}
+
+__global__ void k_climb_select(GnState* state, GnArgs ga, ClimbDev* c, const double* __restrict__ try_H,
+                               const int* __restrict__ try_ok, const double* __restrict__ score, int gn) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  GnState& s = state[0];
+  if (c->done || !s.active) return;
+  const int T = ga.n_try;
+  int best = -1;
+  double bs = s.score;
+  for (int k = 0; k < T; ++k) {
+    if (!try_ok[k]) continue;
+    const double f = score[k];
+    const double v = ga.symmetric ? 0.5 * (f + score[T + k]) : f;
+    if (gd::beats(v, s.score) && (best < 0 || v > bs)) {
+      best = k;
+      bs = v;
+    }
+  }
+  if (best < 0) {
+    if (gn) {
+      s.active = 0;   // the block ends at a step that beats nothing
+      c->step_ok = 1;
+    }
+    return;
+  }
+  for (int q = 0; q < 9; ++q) s.H[q] = try_H[best * 9 + q];
+  s.score = bs;
+  s.moved = 1;
+  c->moved = 1;
+  c->step_ok = 0;
+  ++c->n_rec;
+}
+
+// A block at a pose whose Gauss–Newton step already beat nothing would try the same five poses
+// with the same scores: it is skipped (the host loop runs it and finds nothing).
+__global__ void k_climb_block(GnState* state, const ClimbDev* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) state[0].active = (c->done || c->step_ok) ? 0 : 1;
+}
+
+// iter_end: a move resets the stall count; two iterations in a row without one end the climb
+// (one when no hop is on), as does n_iters. An iteration without a move leaves the pose, and with
+// it the frame, the maps and the tries of the next iteration, unchanged, so the next one would
+// repeat it move for move: it is counted and not run.
+__global__ void k_climb_end(ClimbDev* c, ClimbArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || c->done) return;
+  c->stall = c->moved ? 0 : c->stall + 1;
+  ++c->iter;
+  if (!c->moved && (a.hop || a.hop_fm) && c->stall < 2 && c->iter < a.n_iters) {
+    ++c->iter;
+    c->stall = 2;
+  }
+  if ((!c->moved && (!(a.hop || a.hop_fm) || c->stall >= 2)) || c->iter >= a.n_iters) c->done = 1;
+}
+
+void launch_climb_begin(ClimbDev* c, GnState* s, const ClimbArgs& a, double* hi, bool reset_moved, hipStream_t st) {
+  k_climb_begin<<<1, 64, 0, st>>>(c, s, a, hi, reset_moved ? 1 : 0);
+}
+
+void launch_climb_hop(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, const ClimbArgs& a, const float* res,
+                      bool rs, hipStream_t st) {
+  k_climb_hop<<<1, 64, 0, st>>>(b.pairs, b.state, ga, c, a, res, rs ? 1 : 0, b.try_H, b.try_ok, b.try_jobs);
+}
+
+void launch_climb_try_scores(const GnBuffers& b, const GnArgs& ga, hipStream_t st) {
+  const int nd = (ga.symmetric ? 2 : 1) * ga.n_try;
+  k_gn_try_scores<<<(nd + 63) / 64, 64, 0, st>>>(b.state, ga, b.try_sums, b.try_ok, b.try_score);
+}
+
+void launch_climb_select(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, bool gn, hipStream_t st) {
+  k_climb_select<<<1, 64, 0, st>>>(b.state, ga, c, b.try_H, b.try_ok, b.try_score, gn ? 1 : 0);
+}
+
+void launch_climb_gn_block(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, int blocks_grad, int blocks_try,
+                           hipStream_t st) {
+  const int nd = ga.symmetric ? 2 : 1, T = ga.n_try;
+  k_climb_block<<<1, 64, 0, st>>>(b.state, c);
+  for (int step = 0; step < 3; ++step) {   // gclimb.zig:51 GN_STEPS
+    k_gn_grad_jobs<<<1, 64, 0, st>>>(b.pairs, b.state, ga, b.gjobs);
+    launch_pose_moments_jobs(b.gjobs, nd, blocks_grad, b.part, b.gsums, st);
+    k_gn_coefs<<<1, 64, 0, st>>>(b.pairs, b.state, ga, b.gsums, b.dir_score, b.dir_n, b.tjobs);
+    launch_tangent_jobs(b.tjobs, nd, blocks_grad, b.part, b.tsums, st);
+    k_gn_step<<<1, 64, 0, st>>>(b.pairs, b.state, ga, b.tsums, b.dir_score, b.dir_n, b.step);
+    k_gn_try_jobs<<<(T + 63) / 64, 64, 0, st>>>(b.pairs, b.state, ga, b.step, b.try_H, b.try_ok, b.try_jobs);
+    launch_pose_moments_jobs(b.try_jobs, nd * T, blocks_try, b.part, b.try_sums, st);
+    k_gn_try_scores<<<(nd * T + 63) / 64, 64, 0, st>>>(b.state, ga, b.try_sums, b.try_ok, b.try_score);
+    k_climb_select<<<1, 64, 0, st>>>(b.state, ga, c, b.try_H, b.try_ok, b.try_score, 1);
+  }
+}
+
+void launch_climb_end(ClimbDev* c, const ClimbArgs& a, hipStream_t st) { k_climb_end<<<1, 64, 0, st>>>(c, a); }
 
 }  // namespace zc

@@ -222,8 +222,129 @@ static void pack_rowfft_L(int plan, int side, const float* canvas, const Resampl
 }
 
 
+// Resampled maps: every cell's five values (k×k bilinear taps each), one thread per cell, so the
+// taps are gathered by the whole chip rather than by a pack launch of ch · nx / 16 threads.
+__global__ __launch_bounds__(256) void k_resample_cells(const float* __restrict__ canvas, Resample r, float* cells) {
+  const int64_t nc = (int64_t)r.cw * r.ch;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nc) return;
+  float s[5];
+  cell_values(canvas, r, (int)(t % r.cw), (int)(t / r.cw), s);
+#pragma unroll
+  for (int p = 0; p < 5; ++p) cells[p * nc + t] = s[p];
+}
+
+// The same with a workgroup's 16 × 16 cells reading their canvas footprint from LDS (each canvas
+// pixel loaded once per tile instead of by every tap that touches it): identical arithmetic.
+constexpr int kResTile = 16, kResTilePx = 2048;
+struct ResFoot {
+  int x0, y0, w, h;   // the tile's canvas footprint (clamped pixel indices)
+};
+__host__ __device__ __forceinline__ ResFoot res_footprint(const Resample& r, int cx0, int cy0) {
+  const int cx1 = (cx0 + kResTile < r.cw ? cx0 + kResTile : r.cw) - 1;
+  const int cy1 = (cy0 + kResTile < r.ch ? cy0 + kResTile : r.ch) - 1;
+  const float pxl = ((float)cx0 + 0.5f / (float)r.kx) * r.fx;
+  const float pxh = ((float)cx1 + ((float)(r.kx - 1) + 0.5f) / (float)r.kx) * r.fx;
+  const float pyl = ((float)cy0 + 0.5f / (float)r.ky) * r.fy;
+  const float pyh = ((float)cy1 + ((float)(r.ky - 1) + 0.5f) / (float)r.ky) * r.fy;
+  auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  const int xl = cl((int)floorf(pxl - 0.5f), r.W - 1), xh = cl((int)floorf(pxh - 0.5f) + 1, r.W - 1);
+  const int yl = cl((int)floorf(pyl - 0.5f), r.H - 1), yh = cl((int)floorf(pyh - 0.5f) + 1, r.H - 1);
+  return {xl, yl, xh - xl + 1, yh - yl + 1};
+}
+
+__global__ __launch_bounds__(kResTile * kResTile) void k_resample_tile(const float* __restrict__ canvas, Resample r,
+                                                                       float* cells) {
+  __shared__ float tile[5 * kResTilePx];
+  const int cx0 = blockIdx.x * kResTile, cy0 = blockIdx.y * kResTile;
+  const ResFoot f = res_footprint(r, cx0, cy0);
+  const int64_t n = (int64_t)r.W * r.H, fp = (int64_t)f.w * f.h;
+  const int x = cx0 + (threadIdx.x % kResTile), y = cy0 + (threadIdx.x / kResTile);
+  const int64_t nc = (int64_t)r.cw * r.ch, t = (int64_t)y * r.cw + x;
+  float s[5];
+  if (fp > kResTilePx) {   // a footprint larger than the tile (not launched so): from the canvas
+    if (x >= r.cw || y >= r.ch) return;
+    cell_values(canvas, r, x, y, s);
+#pragma unroll
+    for (int p = 0; p < 5; ++p) cells[p * nc + t] = s[p];
+    return;
+  }
+  for (int q = threadIdx.x; q < 5 * fp; q += kResTile * kResTile) {
+    const int p = q / (int)fp, rem = q - p * (int)fp, yy = rem / f.w, xx = rem - yy * f.w;
+    tile[q] = canvas[p * n + (int64_t)(f.y0 + yy) * r.W + f.x0 + xx];
+  }
+  __syncthreads();
+  if (x >= r.cw || y >= r.ch) return;
+#pragma unroll
+  for (int p = 0; p < 5; ++p) s[p] = 0.0f;
+  for (int j = 0; j < r.ky; ++j) {
+    const float py = ((float)y + ((float)j + 0.5f) / (float)r.ky) * r.fy;
+    for (int i = 0; i < r.kx; ++i) {
+      const float px = ((float)x + ((float)i + 0.5f) / (float)r.kx) * r.fx;
+      // sample_plane's taps, read from the tile
+      const float xs = px - 0.5f, ys = py - 0.5f;
+      const float x0f = floorf(xs), y0f = floorf(ys);
+      const float ax = xs - x0f, ay = ys - y0f;
+      const int ix = (int)x0f, iy = (int)y0f;
+      const int xa = min(max(ix, 0), r.W - 1) - f.x0, xb = min(max(ix + 1, 0), r.W - 1) - f.x0;
+      const int ya = min(max(iy, 0), r.H - 1) - f.y0, yb = min(max(iy + 1, 0), r.H - 1) - f.y0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) {
+        const float* sp = tile + p * fp;
+        const float a = sp[ya * f.w + xa], b = sp[ya * f.w + xb];
+        const float c = sp[yb * f.w + xa], d = sp[yb * f.w + xb];
+        s[p] += (a * (1.0f - ax) + b * ax) * (1.0f - ay) + (c * (1.0f - ax) + d * ax) * ay;
+      }
+    }
+  }
+  const float inv = 1.0f / (float)(r.kx * r.ky);
+#pragma unroll
+  for (int p = 0; p < 5; ++p) cells[p * nc + t] = s[p] * inv;
+}
+
+// Large cells (k×k >= 16 taps): one wave per cell, one lane per tap (coalesced gathers); the tap
+// values meet in LDS and lane p adds plane p's in cell_values' order, so the sums are identical.
+__global__ __launch_bounds__(256) void k_resample_wave(const float* __restrict__ canvas, Resample r, float* cells) {
+  __shared__ float tv[4][5][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nc = (int64_t)r.cw * r.ch, n = (int64_t)r.W * r.H;
+  const int64_t cell = (int64_t)blockIdx.x * 4 + w;
+  const int K = r.kx * r.ky;
+  const bool live = cell < nc;
+  const int x = live ? (int)(cell % r.cw) : 0, y = live ? (int)(cell / r.cw) : 0;
+  if (live && lane < K) {
+    const int i = lane % r.kx, j = lane / r.kx;
+    const float py = ((float)y + ((float)j + 0.5f) / (float)r.ky) * r.fy;
+    const float px = ((float)x + ((float)i + 0.5f) / (float)r.kx) * r.fx;
+#pragma unroll
+    for (int p = 0; p < 5; ++p) tv[w][p][lane] = sample_plane(canvas + p * n, r.W, r.H, px, py);
+  }
+  __syncthreads();
+  if (live && lane < 5) {
+    float acc = 0.0f;
+    for (int q = 0; q < K; ++q) acc += tv[w][lane][q];
+    cells[lane * nc + cell] = acc * (1.0f / (float)K);
+  }
+}
+
+static void launch_resample(const float* canvas, const Resample& r, float* cells, hipStream_t st) {
+  if (r.kx * r.ky >= 16) {
+    k_resample_wave<<<(unsigned)(((int64_t)r.cw * r.ch + 3) / 4), 256, 0, st>>>(canvas, r, cells);
+    return;
+  }
+  // the tile form when every tile's footprint fits its LDS (the largest is a full interior tile)
+  const ResFoot f = res_footprint(r, 0, 0);
+  const int fw = (int)ceilf(kResTile * r.fx) + 4, fh = (int)ceilf(kResTile * r.fy) + 4;
+  if ((int64_t)std::max(fw, f.w) * std::max(fh, f.h) <= kResTilePx) {
+    const dim3 g((r.cw + kResTile - 1) / kResTile, (r.ch + kResTile - 1) / kResTile);
+    k_resample_tile<<<g, kResTile * kResTile, 0, st>>>(canvas, r, cells);
+  } else {
+    k_resample_cells<<<(unsigned)(((int64_t)r.cw * r.ch + 255) / 256), 256, 0, st>>>(canvas, r, cells);
+  }
+}
+
 bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
-                        float2* spec, const float2* tw_x, hipStream_t st) {
+                        float2* spec, const float2* tw_x, hipStream_t st, float* cells) {
   Resample r;
   r.W = W;
   r.H = H;
@@ -234,6 +355,11 @@ bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, i
   r.kx = (int)fminf(fmaxf(ceilf(r.fx - 1e-4f), 1.0f), 8.0f);
   r.ky = (int)fminf(fmaxf(ceilf(r.fy - 1e-4f), 1.0f), 8.0f);
   r.identity = (cw == W && ch == H) ? 1 : 0;
+  if (!r.identity && cells && cw > 0 && ch > 0) {
+    launch_resample(canvas, r, cells, st);
+    canvas = cells;
+    r = Resample{cw, ch, cw, ch, 1, 1, 1, 1.0f, 1.0f};
+  }
 #define CALL(LL) pack_rowfft_L<LL>(plan, side, canvas, r, ny, spec, tw_x, st)
   ZC_POW2_SWITCH(nx, CALL)
 #undef CALL

@@ -368,12 +368,17 @@ void launch_pose_moments(const float* feat_a, int wa, int ha, const float* feat_
   k_pose_moments<<<moments_blocks(wb, hb), 256, 0, st>>>(feat_a, wa, ha, feat_b, wb, hb, p, part);
 }
 
-// Workgroups per pose: about 4 chunks per wave, at most 256, a multiple of 8 — the hardware deals
+// Workgroups per pose: about 4 chunks per wave, at most 256 (more, down to one chunk per wave, for
+// launches of few jobs), a multiple of 8 — the hardware deals
 // workgroups to the 8 XCDs in turn, so chunk x of consecutive poses (the tries of one slice, which
 // read the same features) runs on the same XCD and shares its L2.
-int jobs_blocks(int64_t max_pixels) {
+int jobs_blocks(int64_t max_pixels, int njobs) {
   const int64_t chunks = (max_pixels + kChunkPx - 1) / kChunkPx;
-  const int64_t b = std::max<int64_t>(1, std::min<int64_t>(256, (chunks + 15) / 16));
+  int64_t b = std::max<int64_t>(1, std::min<int64_t>(256, (chunks + 15) / 16));
+  // few jobs (a single pose and its tries): enough workgroups to fill the chip — about 2048 in
+  // all, down to one chunk per wave
+  const int64_t fill = (2048 + std::max(njobs, 1) - 1) / std::max(njobs, 1);
+  b = std::max(b, std::min<int64_t>(fill, std::min<int64_t>(1024, (chunks + 3) / 4)));
   return (int)((b + 7) / 8 * 8);
 }
 

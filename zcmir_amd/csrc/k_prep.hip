@@ -226,15 +226,16 @@ struct H9 {
   double m[9];
 };
 
-__global__ __launch_bounds__(256) void k_warp_stack(const float* __restrict__ stack, int sw, int sh, H9 Hi,
-                                                    float* canvas, int cw, int ch) {
+// one canvas pixel of the warped stack (Hi: canvas → source, 1-based pixels)
+__device__ __forceinline__ void warp_stack_px(const float* __restrict__ stack, int sw, int sh, const double* Hi,
+                                              float* canvas, int cw, int ch) {
   const int gx = blockIdx.x * 16 + (threadIdx.x & 15), gy = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (gx >= cw || gy >= ch) return;
   const int64_t n = (int64_t)cw * ch, pix = (int64_t)gy * cw + gx, sn = (int64_t)sw * sh;
   const double X = gx + 1.0, Y = gy + 1.0;
-  const double qx = Hi.m[0] * X + Hi.m[1] * Y + Hi.m[2];
-  const double qy = Hi.m[3] * X + Hi.m[4] * Y + Hi.m[5];
-  const double qz = Hi.m[6] * X + Hi.m[7] * Y + Hi.m[8];
+  const double qx = Hi[0] * X + Hi[1] * Y + Hi[2];
+  const double qy = Hi[3] * X + Hi[4] * Y + Hi[5];
+  const double qz = Hi[6] * X + Hi[7] * Y + Hi[8];
   bool ok = fabs(qz) >= 1e-8;
   double fx = 0.0, fy = 0.0;
   if (ok) {
@@ -264,6 +265,17 @@ __global__ __launch_bounds__(256) void k_warp_stack(const float* __restrict__ st
     }
     canvas[p * n + pix] = v;
   }
+}
+
+__global__ __launch_bounds__(256) void k_warp_stack(const float* __restrict__ stack, int sw, int sh, H9 Hi,
+                                                    float* canvas, int cw, int ch) {
+  warp_stack_px(stack, sw, sh, Hi.m, canvas, cw, ch);
+}
+
+// the pose read from device memory (a pose that device code chose: the climb's)
+__global__ __launch_bounds__(256) void k_warp_stack_dev(const float* __restrict__ stack, int sw, int sh,
+                                                        const double* __restrict__ Hi, float* canvas, int cw, int ch) {
+  warp_stack_px(stack, sw, sh, Hi, canvas, cw, ch);
 }
 
 __global__ __launch_bounds__(256) void k_paste_stack(const float* __restrict__ stack, int sw, int sh, int ox, int oy,
@@ -319,6 +331,12 @@ void launch_warp_stack(const float* stack, int sw, int sh, const double* Hi, flo
   for (int i = 0; i < 9; ++i) h.m[i] = Hi[i];
   dim3 g((cw + 15) / 16, (ch + 15) / 16);
   k_warp_stack<<<g, 256, 0, st>>>(stack, sw, sh, h, canvas, cw, ch);
+}
+
+void launch_warp_stack_dev(const float* stack, int sw, int sh, const double* Hi_dev, float* canvas, int cw, int ch,
+                           hipStream_t st) {
+  dim3 g((cw + 15) / 16, (ch + 15) / 16);
+  k_warp_stack_dev<<<g, 256, 0, st>>>(stack, sw, sh, Hi_dev, canvas, cw, ch);
 }
 void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, float* canvas, int cw, int ch,
                         hipStream_t st) {

@@ -45,12 +45,16 @@ void launch_acf(const float* feat, int w, int h, double* part, hipStream_t st);
 // ── maps ──
 void launch_warp_stack(const float* stack, int sw, int sh, const double* Hi, float* canvas, int cw, int ch,
                        hipStream_t st);
+// the same with Hi (9 doubles) in device memory
+void launch_warp_stack_dev(const float* stack, int sw, int sh, const double* Hi_dev, float* canvas, int cw, int ch,
+                           hipStream_t st);
 void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, float* canvas, int cw, int ch,
                         hipStream_t st);
 // grid: nx × ny FFT, cw × ch cells resampled from a W × H canvas; writes n_cplx planes of
-// ch rows × nx into spec (plane stride nx·ny). Returns false for an unsupported length.
+// ch rows × nx into spec (plane stride nx·ny). Returns false for an unsupported length. cells
+// (5 · cw · ch floats, optional): where a resampled grid's cell values are gathered first.
 bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
-                        float2* spec, const float2* tw_x, hipStream_t st);
+                        float2* spec, const float2* tw_x, hipStream_t st, float* cells = nullptr);
 // column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
 // lines: columns per workgroup (0: the default for the length)
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
@@ -218,7 +222,38 @@ struct GnBuffers {
 };
 // one iteration for every active pose (kernel launches only: capturable)
 void launch_gn_iteration(const GnBuffers& b, const GnArgs& a, int blocks_grad, int blocks_try, hipStream_t st);
-int jobs_blocks(int64_t max_pixels);
+
+// ── the climb on the device (gclimb.zig:313-489 run; climb.wgsl hop_cand, rs_cand, gn_cands,
+// climb_select, iter_begin / iter_end): pose 0 of a one-pose GnBuffers batch is the climb's pose ──
+struct ClimbDev {
+  int done, moved, stall, iter, n_rec;
+  int step_ok;   // the Gauss–Newton step at the current pose was tried and beat nothing
+};
+struct ClimbArgs {
+  int hop, hop_fm, n_iters, homography;
+  int ox, oy;                 // the batch's canvas origin (the maps' frame)
+  int map_nx, map_ny;         // the shift map's grid
+  double cell_x, cell_y;      // canvas pixels per map cell
+  int rs_n, rs_nth;           // the roto-scale map's grid
+  double rs_dlam, rs_c0, rs_c1;   // its log-radius step and centre (1-based fixed pixels)
+};
+// iteration start: no move yet; hi = the canvas → moving inverse of the pose (the warp's Hi)
+void launch_climb_begin(ClimbDev* c, GnState* s, const ClimbArgs& a, double* hi, bool reset_moved, hipStream_t st);
+// a hop: try slot 0 of pose 0 from the map's peak (res: [peak, index bits, map[0]]); rs: from the
+// roto-scale map's peak, else from the shift map's
+void launch_climb_hop(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, const ClimbArgs& a, const float* res,
+                      bool rs, hipStream_t st);
+// the tries' scores (one-pose batch)
+void launch_climb_try_scores(const GnBuffers& b, const GnArgs& ga, hipStream_t st);
+// the best try that beats the pose by 1e-6 of its score replaces it; gn: a step with no such try
+// ends the Gauss–Newton block (the pose leaves the batch)
+void launch_climb_select(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, bool gn, hipStream_t st);
+// a Gauss–Newton block: up to three steps of five lengths
+void launch_climb_gn_block(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, int blocks_grad, int blocks_try,
+                           hipStream_t st);
+void launch_climb_end(ClimbDev* c, const ClimbArgs& a, hipStream_t st);
+// workgroups per job of a launch of njobs pose jobs of at most max_pixels destination pixels
+int jobs_blocks(int64_t max_pixels, int njobs);
 // 45 sums per job into sums (part: njobs · blocks · 45 doubles)
 void launch_pose_moments_jobs(const PoseJob* jobs, int njobs, int blocks, double* part, double* sums,
                               hipStream_t st);

@@ -22,7 +22,7 @@ constexpr uint32_t kRsMaxN = 2048;   // maxFftN (fft.zig:118-120): the longest r
 
 Engine::RsPlan Engine::rs_map_dir(Side& A, Side& B, const Mat3& Hp, double c0, double c1, double r0_opt,
                                   double r1_opt, bool exact, bool calib, uint32_t cap, DBuf& dst,
-                                  const Frame* frame) {
+                                  const Frame* frame, const double* dev_hi) {
   if (A.mode != B.mode) throw std::runtime_error("FeatureModeMismatch: set both images with the same smi_features");
   const bool three = A.mode == "three";
   const int plan = (exact ? 1 : 0) + (three ? 2 : 0);
@@ -77,7 +77,10 @@ Engine::RsPlan Engine::rs_map_dir(Side& A, Side& B, const Mat3& Hp, double c0, d
   double* rho = rs_tab.as<double>();
   double* cth = rho + p.n_lam;
   double* sth = cth + p.n_th;
-  LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
+  if (dev_hi)
+    LAUNCH("warp_stack", launch_warp_stack_dev(A.stack.as<float>(), A.w, A.h, dev_hi, canvas_a.as<float>(), W, Hh, st));
+  else
+    LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
   LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, fr.ox, fr.oy, canvas_b.as<float>(), W, Hh, st));
   LAUNCH("rs_tables", launch_rs_tables(p.n_th, p.n_lam, p.log0, p.span, rho, cth, sth, st));
   LogPolarArgs la{};
@@ -121,7 +124,7 @@ Engine::RsPlan Engine::rs_map_dir(Side& A, Side& B, const Mat3& Hp, double c0, d
   return p;
 }
 
-void Engine::rs_finish(const RsPlan& p, bool symmetric, bool mirror) {
+void Engine::rs_finish(const RsPlan& p, bool symmetric, bool mirror, bool to_host) {
   const int n = p.n;
   const int64_t nn = (int64_t)n * n;
   const double smin = set.f("rs_smin"), smax = set.f("rs_smax");
@@ -134,6 +137,7 @@ void Engine::rs_finish(const RsPlan& p, bool symmetric, bool mirror) {
   LAUNCH("rs_finish", launch_rs_finish(rs_fwd.as<float>(), symmetric ? rs_inv.as<float>() : nullptr, n, mirror, p.dlam,
                                        lo, hi, rs_out.as<float>(), arg_part.as<float>(), st));
   LAUNCH("argmax_finish", launch_argmax_finish(arg_part.as<float>(), nparts, rs_out.as<float>(), res.as<float>(), st));
+  if (!to_host) return;
   ZC_HIP(hipMemcpyAsync(res_host, res.p, 12, hipMemcpyDeviceToHost, st));
   prof.break_chain();
 }

@@ -129,6 +129,9 @@ static const SettingDef kSettings[] = {
     // this library's: run Gauss-Newton batches (stacks, the search's refinement) on the device, one
     // captured HIP graph per iteration, no host round trip (false: the host-driven loop)
     {"gn_graph", Kind::Bool, "true", nullptr},
+    // this library's: run the climb's decisions on the device, one captured HIP graph per
+    // iteration read back once per iteration (false: the host-driven loop)
+    {"climb_graph", Kind::Bool, "true", nullptr},
     {"hho_auto", Kind::Bool, "true", nullptr},
     {"hho_n", Kind::U32, "100", nullptr},
     {"hho_t", Kind::U32, "20", nullptr},
