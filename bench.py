@@ -439,7 +439,8 @@ def bench_prior(a, np, torch, zcmir_amd):
                      "rot_diff_deg": round(dth, 4), "fell_back": bool(info["fell_back"]),
                      "check_score": round(info["check_score"], 4),
                      "truth_cell_proposed": prior.truth_cell(th[i], sg[i]) in info["cells"],
-                     "truth_dist_px": round(_pose_close(full.H, Hs[i], size, size)[0], 3)})
+                     "truth_dist_px": round(_pose_close(full.H, Hs[i], size, size)[0], 3),
+                     "learned_truth_dist_px": round(_pose_close(lrn.H, Hs[i], size, size)[0], 3)})
     reg.close()
     tf = float(np.mean([r["t_full_s"] for r in rows]))
     tl = float(np.mean([r["t_learned_s"] for r in rows]))
@@ -465,7 +466,7 @@ def main():
     ap.add_argument("--fft-rows", type=int, default=0, help="tiling of the inverse row FFT, 0-3 (0: default)")
     ap.add_argument("--sweep-steps", type=int, default=10, help="timed sweeps on the 128² canvas (0: skip)")
     ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
-    ap.add_argument("--prior-pairs", type=int, default=6, help="time-to-pose pairs, learned vs full (0: skip)")
+    ap.add_argument("--prior-pairs", type=int, default=24, help="time-to-pose pairs, learned vs full (0: skip)")
     ap.add_argument("--refine-runs", type=int, default=3, help="timed climbs on the 1024² pair (0: skip)")
     ap.add_argument("--rs-runs", type=int, default=10, help="timed roto-scale maps on a 1024² pair (0: skip)")
     ap.add_argument("--sweep-batch", type=int, default=0, help="sweep candidates per batch (0: 256, the default)")
