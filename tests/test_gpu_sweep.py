@@ -74,6 +74,36 @@ def test_sweep_full_resolution_candidate(reg_factory):
     assert (peaks[0]["sx"], peaks[0]["sy"]) == (r["peaks"][0]["sx"], r["peaks"][0]["sy"])
 
 
+@pytest.mark.parametrize("size", [512, 1024])
+def test_sweep_smooth_planes_match_powers_of_two(reg_factory, size):
+    """Full-resolution candidates whose windows need 1315 (512² pair) or 2630 (1024² pair) points
+    run on planes of 1536 or 3072 points (radix-3 stages) instead of 2048 or 4096: a linear
+    correlation does not depend on the plane length, so every candidate's best score and best lag
+    match the power-of-two sweep (itself checked against the oracle above)."""
+    mv, fx, _ = synth.rotscale_pair((size, size), theta_deg=37.0, scale=1.12, seed=9)
+    runs = {}
+    for smooth in (False, True):
+        reg = reg_factory(half=False, sw_full=True, sweep_smooth=smooth, sw_rot_range=True, sw_rot_lo=33.0,
+                          sw_rot_hi=41.0, sw_nth=3, sw_smin=1.1, sw_smax=1.4, sw_ns=2)
+        reg.set_images(mv, fx, preprocess=False)
+        info, peaks, best = reg.sweep()
+        runs[smooth] = (info, peaks, best, reg.sweep_map().shape)
+    (ip, pp, bp, mp), (is_, ps, bs, ms) = runs[False], runs[True]
+    want = 1536 if size == 512 else 3072
+    assert ip["n"] == 4 * want // 3 and mp == (4 * want // 3,) * 2, (ip["n"], mp)
+    assert is_["n"] == want and ms == (want, want), (is_["n"], ms)
+    assert np.abs(bs - bp).max() <= TOL * bp.max()
+    def firsts(lst):
+        out = {}
+        for p in lst:
+            out.setdefault(int(p["cand"]), p)
+        return out
+    fp, fs = firsts(pp), firsts(ps)
+    assert set(fp) == set(fs)
+    for c in fp:
+        assert (fp[c]["sx"], fp[c]["sy"]) == (fs[c]["sx"], fs[c]["sy"]), c
+
+
 def test_sweep_landscape_matches_oracle(reg_factory):
     """A 12 θ × 3 σ sweep: every candidate's best score and best lag as the oracle's."""
     mv, fx, _ = synth.rotscale_pair((128, 128), theta_deg=37.0, scale=1.12, seed=6)

@@ -76,6 +76,7 @@ zc::SweepOpts sweep_opts(Engine* e, zc::SweepGrid& gv) {
   o.gv = &gv;
   o.grid = (g == 32 || g == 64 || g == 128 || g == 256) ? g : 128;
   o.full = s.b("sw_full");
+  o.smooth = s.b("sweep_smooth");
   o.mode = s.b("exact") ? 1 : 0;
   o.n_peak = 4;
   const int tb = zc::SweepTune::batch;
@@ -596,7 +597,7 @@ void zc_fft_tune_wg(uint32_t, uint32_t, uint32_t) {}
 int32_t zc_fft_tiling(void* p, uint32_t L, uint32_t* cols) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "fft_tiling", [&] {
-    if (L < 32 || L > 4096 || (L & (L - 1))) throw std::runtime_error("FftLength: a power of two, 32..4096");
+    if (!zc::fft_length_ok((int)L)) throw std::runtime_error("FftLength: a power of two 32..4096, 1536 or 3072");
     *cols = (uint32_t)e->col_lines((int)L);
   });
 }

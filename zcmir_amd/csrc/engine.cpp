@@ -250,7 +250,7 @@ int Engine::autotune_cols(int L) {
   };
   double t_best = time(0);
   int c_best = 0;
-  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47, 48, 49}) {
+  for (int lines : {1, 2, 4, 8, 16, 9, 17, 33, 68, 72, 80, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53}) {
     const double t = time(lines);
     if (t < 0.97 * t_best) {
       t_best = t;
@@ -825,7 +825,7 @@ double Engine::score_from_moments(const double* s, bool exact, double ridge) {
 }
 
 void Engine::fft2(float* data, int n, int planes, bool inverse) {
-  if (n > kMaxFft || (n & (n - 1)) || n < 32) throw std::runtime_error("FftLength: a power of two, 32..4096");
+  if (!fft_length_ok(n)) throw std::runtime_error("FftLength: a power of two 32..4096, 1536 or 3072");
   const size_t bytes = (size_t)planes * n * n * 8;
   work.ensure(bytes);
   ZC_HIP(hipMemcpyAsync(work.p, data, bytes, hipMemcpyHostToDevice, st));

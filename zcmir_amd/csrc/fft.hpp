@@ -53,6 +53,22 @@ __device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2&
 template <int S>
 __device__ __forceinline__ void dft4v(float2* v) { dft4<S>(v[0], v[1], v[2], v[3]); }
 
+// DFT3 with w = e^{S 2πi/3} = −1/2 + S i √3/2: X1 = a − (b + c)/2 + S i (√3/2)(b − c), X2 its
+// conjugate-rotated twin.
+template <int S>
+__device__ __forceinline__ void dft3(float2& a, float2& b, float2& c) {
+  constexpr float H = 0.86602540378443864676f;   // √3/2
+  const float2 t1 = cadd(b, c);
+  const float2 t2 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+  const float2 d = mul_si<S>(make_float2(H * (b.x - c.x), H * (b.y - c.y)));
+  a = cadd(a, t1);
+  b = cadd(t2, d);
+  c = csub(t2, d);
+}
+
+template <int S>
+__device__ __forceinline__ void dft3v(float2* v) { dft3<S>(v[0], v[1], v[2]); }
+
 template <int S>
 __device__ __forceinline__ void dft8(float2* v) {
   constexpr float C = 0.70710678118654752440f;

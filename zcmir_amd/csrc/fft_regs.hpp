@@ -55,38 +55,107 @@ __device__ __forceinline__ void dft16(float2* v) {
   }
 }
 
+// Mixed radix N = N1 · N2 (Cooley–Tukey, input n2 + N2 n1, output k1 + N1 k2): DFT_N1 over n1 for
+// each n2, twiddle w_N^(n2 k1), DFT_N2 over n2 for each k1. 6 = 2 x 3 and 12 = 4 x 3 for lines of
+// 3 · 2^k points; w_N^m = cos(2πm/N) + S i sin(2πm/N).
+template <int S>
+__device__ __forceinline__ void dft6(float2* v) {
+  constexpr float H = 0.86602540378443864676f;   // √3/2
+  auto tw = [](float2 a, float c, float s) { return make_float2(a.x * c - a.y * s, a.x * s + a.y * c); };
+  float2 y[3][2];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    const float2 a = v[n2], b = v[n2 + 3];
+    y[n2][0] = cadd(a, b);
+    y[n2][1] = csub(a, b);
+  }
+  y[1][1] = tw(y[1][1], 0.5f, S * H);    // w6^1
+  y[2][1] = tw(y[2][1], -0.5f, S * H);   // w6^2
+#pragma unroll
+  for (int k1 = 0; k1 < 2; ++k1) {
+    float2 a = y[0][k1], b = y[1][k1], c = y[2][k1];
+    dft3<S>(a, b, c);
+    v[k1] = a;
+    v[k1 + 2] = b;
+    v[k1 + 4] = c;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void dft12(float2* v) {
+  constexpr float H = 0.86602540378443864676f;   // √3/2
+  auto tw = [](float2 a, float c, float s) { return make_float2(a.x * c - a.y * s, a.x * s + a.y * c); };
+  float2 y[3][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    float2 a0 = v[n2], a1 = v[n2 + 3], a2 = v[n2 + 6], a3 = v[n2 + 9];
+    dft4<S>(a0, a1, a2, a3);
+    y[n2][0] = a0;
+    y[n2][1] = a1;
+    y[n2][2] = a2;
+    y[n2][3] = a3;
+  }
+  y[1][1] = tw(y[1][1], H, S * 0.5f);       // w12^1
+  y[1][2] = tw(y[1][2], 0.5f, S * H);       // w12^2
+  y[1][3] = mul_si<S>(y[1][3]);             // w12^3
+  y[2][1] = tw(y[2][1], 0.5f, S * H);       // w12^2
+  y[2][2] = tw(y[2][2], -0.5f, S * H);      // w12^4
+  y[2][3] = make_float2(-y[2][3].x, -y[2][3].y);   // w12^6
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 a = y[0][k1], b = y[1][k1], c = y[2][k1];
+    dft3<S>(a, b, c);
+    v[k1] = a;
+    v[k1 + 4] = b;
+    v[k1 + 8] = c;
+  }
+}
+
 template <int R, int S>
 __device__ __forceinline__ void dftr(float2* v) {
   if constexpr (R == 16) dft16<S>(v);
+  else if constexpr (R == 12) dft12<S>(v);
   else if constexpr (R == 8) dft8<S>(v);
+  else if constexpr (R == 6) dft6<S>(v);
   else if constexpr (R == 4) dft4v<S>(v);
+  else if constexpr (R == 3) dft3v<S>(v);
   else dft2<S>(v);
 }
 
-// Radices of a length-L line when each thread holds M points: the largest radix ≤ 16 dividing
-// both M and what is left, with a trailing (16, 2) / (8, 2) rebalanced to (8, 4) / (4, 4).
+// Radices of a length-L line (2^a or 3 · 2^a) when each thread holds M points: the largest radix
+// of {16, 12, 8, 6, 4, 3, 2} dividing both M and what is left, with a trailing (16, 2) / (8, 2)
+// rebalanced to (8, 4) / (4, 4). A line of 3 · 2^a points needs M divisible by 3.
 template <int L, int M>
 struct RPlan {
   struct Arr {
     int r[16];
     int n;
+    bool ok;
   };
   static constexpr Arr make() {
     Arr a{};
+    constexpr int cand[7] = {16, 12, 8, 6, 4, 3, 2};
     int rem = L;
-    while (rem > 1) {
-      int R = 16;
-      while (R > 2 && (M % R != 0 || rem % R != 0)) R >>= 1;
+    a.ok = true;
+    while (rem > 1 && a.n < 16) {
+      int R = 0;
+      for (int i = 0; i < 7 && !R; ++i)
+        if (M % cand[i] == 0 && rem % cand[i] == 0) R = cand[i];
+      if (!R) {
+        a.ok = false;
+        break;
+      }
       a.r[a.n++] = R;
       rem /= R;
     }
-    if (a.n >= 2 && a.r[a.n - 1] == 2 && a.r[a.n - 2] >= 8 && M % 4 == 0) {
+    if (a.n >= 2 && a.r[a.n - 1] == 2 && a.r[a.n - 2] >= 8 && (a.r[a.n - 2] & (a.r[a.n - 2] - 1)) == 0 && M % 4 == 0) {
       a.r[a.n - 2] /= 2;
       a.r[a.n - 1] = 4;
     }
     return a;
   }
   static constexpr Arr A = make();
+  static_assert(A.ok, "no radix plan: the points per thread must hold every odd factor of the line");
   static constexpr int stages = A.n;
   static constexpr int radix(int s) { return A.r[s]; }
   static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * A.r[s - 1]; }

@@ -114,6 +114,18 @@ static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const 
 
 template <int L>
 static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
+  if constexpr (L % 3 == 0) {
+    // lines of 3 · 2^k points (the sweep's planes): the default tiling, plans without rectangle sums
+    using X = RowX<L, 0>;
+    dim3 g((ny + X::LINES - 1) / X::LINES, kPlans[plan & 3].n_out * batch);
+    switch (plan & 3) {
+      case 0: k_rowifft_extract<L, 0, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+      case 1: k_rowifft_extract<L, 1, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+      case 2: k_rowifft_extract<L, 2, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+      default: k_rowifft_extract<L, 3, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    }
+    return;
+  }
   if constexpr (L >= 256) {
     switch (FftRowTiling::v) {
       case 1: rowifft_LV<L, 1>(plan, work, corr, ny, tw, st, batch); return;
@@ -128,8 +140,9 @@ static void rowifft_L(int plan, const float2* work, float* corr, int ny, const f
 
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
                             hipStream_t st, int batch) {
+  if (nx % 3 == 0 && plan > 3) return false;
 #define CALL(LL) rowifft_L<LL>(plan, work, corr, ny, tw_x, st, batch)
-  ZC_POW2_SWITCH(nx, CALL)
+  ZC_FFT_SWITCH(nx, CALL)
 #undef CALL
   return true;
 }

@@ -151,11 +151,10 @@ __global__ __launch_bounds__(256) void k_combine_sweep(const float* __restrict__
 // the lags t ≡ c (mod 256) and keeps its largest 3×3 local maximum (periodic neighbours; ties:
 // the first lag); the n_peak best classes are the peaks. Split over chunks of lags: part holds
 // (value, lag) per (entry, chunk, class).
-// nx, ny: powers of two (lx = log2 nx), so the periodic neighbours are masks.
-__global__ __launch_bounds__(256) void k_peaks_part(const float* __restrict__ ns, int lx, int ny, int64_t chunk,
+// Periodic neighbours wrap at nx and ny (any lengths; maps stay below 2^31 lags).
+__global__ __launch_bounds__(256) void k_peaks_part(const float* __restrict__ ns, int nx, int ny, int64_t chunk,
                                                     float2* part) {
   const int b = blockIdx.y, lid = threadIdx.x;
-  const int nx = 1 << lx, mx = nx - 1, my = ny - 1;
   const int64_t nn = (int64_t)nx * ny;
   const float* m = ns + (int64_t)b * nn;
   const int64_t t0 = (int64_t)blockIdx.x * chunk, t1 = min(nn, t0 + chunk);
@@ -164,15 +163,17 @@ __global__ __launch_bounds__(256) void k_peaks_part(const float* __restrict__ ns
   for (int64_t t = t0 + lid; t < t1; t += 256) {
     const float v = m[t];
     if (v <= bv) continue;
-    const int y = (int)(t >> lx), x = (int)t & mx;
+    const int y = (int)((uint32_t)t / (uint32_t)nx), x = (int)((uint32_t)t - (uint32_t)y * (uint32_t)nx);
+    const int xs[3] = {x == 0 ? nx - 1 : x - 1, x, x == nx - 1 ? 0 : x + 1};
+    const int ys[3] = {y == 0 ? ny - 1 : y - 1, y, y == ny - 1 ? 0 : y + 1};
     bool ok = true;
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int64_t row = (int64_t)((y + dy) & my) << lx;
+    for (int dy = 0; dy < 3; ++dy) {
+      const int64_t row = (int64_t)ys[dy] * nx;
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) {
-        if (dx == 0 && dy == 0) continue;
-        ok &= !(m[row + ((x + dx) & mx)] > v);
+      for (int dx = 0; dx < 3; ++dx) {
+        if (dx == 1 && dy == 1) continue;
+        ok &= !(m[row + xs[dx]] > v);
       }
     }
     if (ok) {
@@ -248,7 +249,7 @@ bool launch_pack_sweep(int plan, const float4* il, const SweepPose* poses, int b
                        float2* spec, const float2* tw_x, hipStream_t st) {
   if (u.win_y <= 0) return true;
 #define CALL(LL) pack_sweep_L<LL>(plan, il, poses, batch, u, spec, tw_x, st)
-  ZC_POW2_SWITCH(nx, CALL)
+  ZC_FFT_SWITCH(nx, CALL)
 #undef CALL
   return true;
 }
@@ -272,9 +273,7 @@ void launch_peaks(const float* ns, int nx, int ny, int batch, int n_peak, float2
   const int64_t nn = (int64_t)nx * ny;
   const int chunks = peaks_chunks(nn);
   const int64_t chunk = ((nn + chunks - 1) / chunks + 255) / 256 * 256;
-  int lx = 0;
-  while ((1 << lx) < nx) ++lx;
-  k_peaks_part<<<dim3(chunks, batch), 256, 0, st>>>(ns, lx, ny, chunk, part);
+  k_peaks_part<<<dim3(chunks, batch), 256, 0, st>>>(ns, nx, ny, chunk, part);
   k_peaks_final<<<batch, 256, 0, st>>>(part, chunks, n_peak, peaks);
 }
 

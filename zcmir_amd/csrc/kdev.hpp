@@ -108,11 +108,11 @@ __device__ __forceinline__ float real_plane(int pl, const float* s) {
 }
 
 // Row kernels: LINES rows of length L per workgroup, M = 16 points per thread (L >= 1024: one
-// row, L / 16 threads).
+// row, L / 16 threads); lines of 3 · 2^k points (1536, 3072) take 12 points per thread.
 template <int L>
 struct RowT {
   static constexpr int LINES = L >= 1024 ? 1 : 1024 / L;
-  static constexpr int M = 16;
+  static constexpr int M = L % 3 == 0 ? 12 : 16;
   static constexpr int T = LINES * L / M;
   static constexpr int STR = padded_len(L);
 };
@@ -130,7 +130,9 @@ struct RowT {
 template <int L, int V>
 struct RowX {
   static constexpr bool WIDE = V == 1 || (V == 0 && L >= 2048);
-  static constexpr int M = WIDE ? 32 : (V == 2 || (V == 0 && L == 256)) ? 8 : 16;
+  // lines of 3 · 2^k points: 48, 12 or 24 points per thread in place of 32, 8 or 16
+  static constexpr int M = L % 3 == 0 ? (WIDE ? 48 : V == 2 ? 12 : 24)
+                                      : (WIDE ? 32 : (V == 2 || (V == 0 && L == 256)) ? 8 : 16);
   static constexpr int TWO = (V == 4 || (V == 0 && L == 1024)) ? 2 : 1;
   static constexpr int MIN_LINES = (64 * M + L - 1) / L;
   static constexpr int LINES =
@@ -272,6 +274,22 @@ __device__ __forceinline__ double lag_score(CF&& C, int64_t t, double floor_n, c
   return score;
 }
 
+
+// The sweep's and the generic FFT's lengths: the powers of two and 3 · 2^k for 1536 and 3072.
+#define ZC_FFT_SWITCH(L, CALL)  \
+  switch (L) {                  \
+    case 32: CALL(32); break;   \
+    case 64: CALL(64); break;   \
+    case 128: CALL(128); break; \
+    case 256: CALL(256); break; \
+    case 512: CALL(512); break; \
+    case 1024: CALL(1024); break; \
+    case 1536: CALL(1536); break; \
+    case 2048: CALL(2048); break; \
+    case 3072: CALL(3072); break; \
+    case 4096: CALL(4096); break; \
+    default: return false;      \
+  }
 
 #define ZC_POW2_SWITCH(L, CALL) \
   switch (L) {                  \

@@ -37,7 +37,15 @@ std::array<uint32_t, 2> cells_of(double a, double b, double o0, double c, int64_
   return {(uint32_t)lo, (uint32_t)hi};
 }
 
-uint32_t plane_len(uint32_t need) { return pow2_ceil(std::max<uint32_t>(need, 32)); }
+// The shortest plane the kernels take that holds `need` points: a power of two, or with `smooth`
+// 1536 or 3072 (3 · 2^k, the reference's ladder lengths in that range: search.zig:180-188).
+uint32_t plane_len(uint32_t need, bool smooth) {
+  uint32_t n = pow2_ceil(std::max<uint32_t>(need, 32));
+  if (smooth)
+    for (uint32_t s3 : {1536u, 3072u})
+      if (s3 >= need && s3 < n) n = s3;
+  return n;
+}
 
 struct Cand {
   uint32_t it, is, ik, ia;
@@ -116,7 +124,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
       const auto mcy = ok ? cells_of(y0, y1, oy, cell, G) : std::array<uint32_t, 2>{0, grid};
       Cand& c = cands[ci++];
       c = {i, j, ik, ia, gv.thetas[i], gv.sigmas[j], H, mcx[0], mcy[0],
-           plane_len(mcx[1] - mcx[0] + Fx - 1), plane_len(mcy[1] - mcy[0] + Fy - 1), 1};
+           plane_len(mcx[1] - mcx[0] + Fx - 1, o.smooth), plane_len(mcy[1] - mcy[0] + Fy - 1, o.smooth), 1};
       // box-filter taps per cell side from the candidate's smallest stretch
       c.taps = (int)std::min(6.0, std::max(1.0, std::ceil(cell / (sv2(L)[1] * s0) - 1e-3)));
       if (c.nx > (uint32_t)kMaxFft || c.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
