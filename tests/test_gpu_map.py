@@ -49,8 +49,8 @@ def test_features_match_oracle(reg_factory):
             assert np.abs(f - p["feat"]).max() < 2e-4, mode
 
 
-@pytest.mark.parametrize("n,planes", [(32, 3), (64, 2), (256, 2), (1024, 1), (1536, 1), (2048, 1), (3072, 1),
-                                      (4096, 1)])
+@pytest.mark.parametrize("n,planes", [(32, 3), (64, 2), (256, 2), (1024, 1), (1536, 1), (2048, 1), (2304, 1),
+                                      (3072, 1), (4096, 1)])
 def test_fft2_matches_numpy(reg_factory, n, planes):
     rng = np.random.default_rng(n)
     x = (rng.standard_normal((planes, n, n)) + 1j * rng.standard_normal((planes, n, n))).astype(np.complex64)
@@ -167,7 +167,7 @@ def test_errors_fail_loudly(reg_factory):
         reg.configure(nope=1)
 
 
-@pytest.mark.parametrize("n", [256, 1024, 1536, 2048, 3072, 4096])
+@pytest.mark.parametrize("n", [256, 1024, 1536, 2048, 2304, 3072, 4096])
 def test_fft2_every_column_tiling(reg_factory, n):
     """Each columns-per-workgroup setting of the column FFT (zc_fft_tune) computes the same FFT."""
     import zcmir_amd

@@ -597,7 +597,7 @@ void zc_fft_tune_wg(uint32_t, uint32_t, uint32_t) {}
 int32_t zc_fft_tiling(void* p, uint32_t L, uint32_t* cols) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "fft_tiling", [&] {
-    if (!zc::fft_length_ok((int)L)) throw std::runtime_error("FftLength: a power of two 32..4096, 1536 or 3072");
+    if (!zc::fft_length_ok((int)L)) throw std::runtime_error("FftLength: a power of two 32..4096, 1536, 2304 or 3072");
     *cols = (uint32_t)e->col_lines((int)L);
   });
 }

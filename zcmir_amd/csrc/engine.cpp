@@ -825,7 +825,7 @@ double Engine::score_from_moments(const double* s, bool exact, double ridge) {
 }
 
 void Engine::fft2(float* data, int n, int planes, bool inverse) {
-  if (!fft_length_ok(n)) throw std::runtime_error("FftLength: a power of two 32..4096, 1536 or 3072");
+  if (!fft_length_ok(n)) throw std::runtime_error("FftLength: a power of two 32..4096, 1536, 2304 or 3072");
   const size_t bytes = (size_t)planes * n * n * 8;
   work.ensure(bytes);
   ZC_HIP(hipMemcpyAsync(work.p, data, bytes, hipMemcpyHostToDevice, st));

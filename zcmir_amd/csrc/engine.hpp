@@ -190,7 +190,7 @@ struct SweepOpts {
   const SweepGrid* gv;
   uint32_t grid = 128;   // canvas cells per side
   bool full = false;     // one cell per fixed pixel (grid = ceil(D))
-  bool smooth = true;    // planes of 1536 and 3072 points where they are the shortest that fit
+  bool smooth = true;    // planes of 1536, 2304 or 3072 points where they are the shortest that fit
   uint32_t mode = 1;     // 0 global, 1 exact SMI
   uint32_t n_peak = 4;
   uint32_t batch = kSweepBatch;   // the most candidates per batch

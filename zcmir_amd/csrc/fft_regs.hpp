@@ -57,7 +57,7 @@ __device__ __forceinline__ void dft16(float2* v) {
 
 // Mixed radix N = N1 · N2 (Cooley–Tukey, input n2 + N2 n1, output k1 + N1 k2): DFT_N1 over n1 for
 // each n2, twiddle w_N^(n2 k1), DFT_N2 over n2 for each k1. 6 = 2 x 3 and 12 = 4 x 3 for lines of
-// 3 · 2^k points; w_N^m = cos(2πm/N) + S i sin(2πm/N).
+// 2^a·3^b points; w_N^m = cos(2πm/N) + S i sin(2πm/N).
 template <int S>
 __device__ __forceinline__ void dft6(float2* v) {
   constexpr float H = 0.86602540378443864676f;   // √3/2
@@ -122,9 +122,9 @@ __device__ __forceinline__ void dftr(float2* v) {
   else dft2<S>(v);
 }
 
-// Radices of a length-L line (2^a or 3 · 2^a) when each thread holds M points: the largest radix
-// of {16, 12, 8, 6, 4, 3, 2} dividing both M and what is left, with a trailing (16, 2) / (8, 2)
-// rebalanced to (8, 4) / (4, 4). A line of 3 · 2^a points needs M divisible by 3.
+// Radices of a length-L line (2^a·3^b) when each thread holds M points: the largest radix of
+// {16, 12, 8, 6, 4, 3, 2} dividing both M and what is left, with a trailing (16, 2) / (8, 2)
+// rebalanced to (8, 4) / (4, 4). A line with a factor 3 needs M divisible by 3.
 template <int L, int M>
 struct RPlan {
   struct Arr {

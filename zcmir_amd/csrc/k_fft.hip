@@ -28,7 +28,7 @@ __global__ __launch_bounds__(RowT<L>::T) void k_rowfft(float2* data, int rows, f
 // 16 points per thread; logical blocks remapped so column neighbours share an XCD's L2.
 template <int L, int C>
 struct ColT {
-  static constexpr int M = L % 3 == 0 ? 12 : 16;   // 3 · 2^k points: 12 per thread
+  static constexpr int M = L % 3 == 0 ? 12 : 16;   // 2^a·3^b points: 12 per thread
   static constexpr int T = C * L / M;
   static constexpr int STR = line_stride(L, C);
   static constexpr bool fits = C * STR * 8 <= 160 * 1024 && T <= 1024 && T >= 16;
@@ -37,12 +37,12 @@ struct ColT {
 // lines (columns) per workgroup of the column FFT: the default, or a tuned value that fits
 // the widest row segments whose LDS fits (measured best at 2048: 8 columns, 1 workgroup per CU)
 inline int col_lines_default(int L) {
-  if (L % 3 == 0) return L >= 3072 ? 4 : 8;
+  if (L % 3 == 0) return L >= 2304 ? 4 : 8;
   return L >= 4096 ? 4 : (L >= 2048 ? 8 : 16);
 }
 static_assert(ColT<32, 16>::fits && ColT<64, 16>::fits && ColT<128, 16>::fits && ColT<256, 16>::fits &&
                   ColT<512, 16>::fits && ColT<1024, 16>::fits && ColT<1536, 8>::fits && ColT<2048, 8>::fits &&
-                  ColT<3072, 4>::fits && ColT<4096, 4>::fits,
+                  ColT<2304, 4>::fits && ColT<3072, 4>::fits && ColT<4096, 4>::fits,
               "every default column tiling must be launchable");
 
 template <int L, int S, int C>
@@ -164,7 +164,7 @@ struct ColW {
   static constexpr int STR = line_stride(L, C);
   static constexpr bool fits =
       L >= 1024 && C * STR * (SPLIT ? 4 : 8) <= 160 * 1024 && T <= 1024 && T >= 64 && (C * L) % M == 0 &&
-      (L % 3 != 0) == (M % 3 != 0) &&   // a line of 3 · 2^k points needs M divisible by 3, and only it
+      (L % 3 != 0) == (M % 3 != 0) &&   // a line of 2^a·3^b points needs M divisible by 3, and only it
       (M < 32 || T <= 512) &&   // 32 complex points per lane need more than the 128 VGPRs of a 1024-lane block
       (M < 64 || T == 512);     // 64 points: 128 VGPRs of data, two waves per SIMD
 };
@@ -247,7 +247,7 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     case 47: return colfft_w_LC<L, S, 2, 32, false>(data, planes, nx, rows_in, scale, tw, st);
     case 48: return colfft_w_LC<L, S, 16, 64, true>(data, planes, nx, rows_in, scale, tw, st);
     case 49: return colfft_w_LC<L, S, 8, 64, true>(data, planes, nx, rows_in, scale, tw, st);
-    // lines of 3 · 2^k points: 50: 8x48, 51: 4x48, 52: 8x24, 53: 4x24 (split exchange)
+    // lines of 2^a·3^b points: 50: 8x48, 51: 4x48, 52: 8x24, 53: 4x24 (split exchange)
     case 50: return colfft_w_LC<L, S, 8, 48, true>(data, planes, nx, rows_in, scale, tw, st);
     case 51: return colfft_w_LC<L, S, 4, 48, true>(data, planes, nx, rows_in, scale, tw, st);
     case 52: return colfft_w_LC<L, S, 8, 24, true>(data, planes, nx, rows_in, scale, tw, st);

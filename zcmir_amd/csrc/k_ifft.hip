@@ -115,7 +115,7 @@ static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const 
 template <int L>
 static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
   if constexpr (L % 3 == 0) {
-    // lines of 3 · 2^k points (the sweep's planes): the default tiling, plans without rectangle sums
+    // lines of 2^a·3^b points (the sweep's planes): the default tiling, plans without rectangle sums
     using X = RowX<L, 0>;
     dim3 g((ny + X::LINES - 1) / X::LINES, kPlans[plan & 3].n_out * batch);
     switch (plan & 3) {

@@ -108,7 +108,7 @@ __device__ __forceinline__ float real_plane(int pl, const float* s) {
 }
 
 // Row kernels: LINES rows of length L per workgroup, M = 16 points per thread (L >= 1024: one
-// row, L / 16 threads); lines of 3 · 2^k points (1536, 3072) take 12 points per thread.
+// row, L / 16 threads); lines of 2^a·3^b points (1536, 2304, 3072) take 12 per thread.
 template <int L>
 struct RowT {
   static constexpr int LINES = L >= 1024 ? 1 : 1024 / L;
@@ -130,7 +130,7 @@ struct RowT {
 template <int L, int V>
 struct RowX {
   static constexpr bool WIDE = V == 1 || (V == 0 && L >= 2048);
-  // lines of 3 · 2^k points: 48, 12 or 24 points per thread in place of 32, 8 or 16
+  // lines of 2^a·3^b points: 48, 12 or 24 points per thread in place of 32, 8 or 16
   static constexpr int M = L % 3 == 0 ? (WIDE ? 48 : V == 2 ? 12 : 24)
                                       : (WIDE ? 32 : (V == 2 || (V == 0 && L == 256)) ? 8 : 16);
   static constexpr int TWO = (V == 4 || (V == 0 && L == 1024)) ? 2 : 1;
@@ -275,7 +275,9 @@ __device__ __forceinline__ double lag_score(CF&& C, int64_t t, double floor_n, c
 }
 
 
-// The sweep's and the generic FFT's lengths: the powers of two and 3 · 2^k for 1536 and 3072.
+// The sweep's and the generic FFT's lengths: the powers of two, and the 2^a·3^b lengths 1536,
+// 2304 and 3072 between 1024 and 4096 (2592 = 2^5·3^4 was measured slower per candidate than
+// 3072: five radix stages, 432-lane column workgroups).
 #define ZC_FFT_SWITCH(L, CALL)  \
   switch (L) {                  \
     case 32: CALL(32); break;   \
@@ -286,6 +288,7 @@ __device__ __forceinline__ double lag_score(CF&& C, int64_t t, double floor_n, c
     case 1024: CALL(1024); break; \
     case 1536: CALL(1536); break; \
     case 2048: CALL(2048); break; \
+    case 2304: CALL(2304); break; \
     case 3072: CALL(3072); break; \
     case 4096: CALL(4096); break; \
     default: return false;      \

@@ -252,8 +252,10 @@ void launch_climb_select(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, bool
 void launch_climb_gn_block(const GnBuffers& b, const GnArgs& ga, ClimbDev* c, int blocks_grad, int blocks_try,
                            hipStream_t st);
 void launch_climb_end(ClimbDev* c, const ClimbArgs& a, hipStream_t st);
-// lengths the FFT kernels take: powers of two from 32 to 4096, and 1536 and 3072 (3 · 2^k)
-inline bool fft_length_ok(int n) { return (n >= 32 && n <= 4096 && (n & (n - 1)) == 0) || n == 1536 || n == 3072; }
+// lengths the FFT kernels take: powers of two from 32 to 4096, and 1536, 2304 and 3072
+inline bool fft_length_ok(int n) {
+  return (n >= 32 && n <= 4096 && (n & (n - 1)) == 0) || n == 1536 || n == 2304 || n == 3072;
+}
 // workgroups per job of a launch of njobs pose jobs of at most max_pixels destination pixels
 int jobs_blocks(int64_t max_pixels, int njobs);
 // 45 sums per job into sums (part: njobs · blocks · 45 doubles)

@@ -126,8 +126,8 @@ static const SettingDef kSettings[] = {
     {"sw_full", Kind::Bool, "false", nullptr},
     // this library's: replay the sweep's launches from a captured HIP graph (eager while profiling)
     {"sweep_graph", Kind::Bool, "true", nullptr},
-    // this library's: sweep planes of 1536 or 3072 points (3 · 2^k) where they are shorter than the
-    // next power of two (false: powers of two only); results do not depend on it
+    // this library's: sweep planes of 1536, 2304 or 3072 points (2^a·3^b) where they are
+    // shorter than the next power of two (false: powers of two only); results do not depend on it
     {"sweep_smooth", Kind::Bool, "true", nullptr},
     // this library's: run Gauss-Newton batches (stacks, the search's refinement) on the device, one
     // captured HIP graph per iteration, no host round trip (false: the host-driven loop)

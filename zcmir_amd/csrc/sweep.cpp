@@ -38,11 +38,12 @@ std::array<uint32_t, 2> cells_of(double a, double b, double o0, double c, int64_
 }
 
 // The shortest plane the kernels take that holds `need` points: a power of two, or with `smooth`
-// 1536 or 3072 (3 · 2^k, the reference's ladder lengths in that range: search.zig:180-188).
+// 1536, 2304 or 3072 (2^a·3^b lengths of the reference's ladder in that range,
+// search.zig:180-188).
 uint32_t plane_len(uint32_t need, bool smooth) {
   uint32_t n = pow2_ceil(std::max<uint32_t>(need, 32));
   if (smooth)
-    for (uint32_t s3 : {1536u, 3072u})
+    for (uint32_t s3 : {1536u, 2304u, 3072u})
       if (s3 >= need && s3 < n) n = s3;
   return n;
 }
