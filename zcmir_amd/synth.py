@@ -28,12 +28,17 @@ def _texture(h: int, w: int, rng: np.random.Generator, n_blobs: int = 20) -> np.
 
     f = gaussian_filter(rng.standard_normal((h, w)), 4.0, mode="wrap")
     f /= max(f.std(), 1e-12)
-    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
     for _ in range(n_blobs):
         cy, cx = rng.uniform(0, h), rng.uniform(0, w)
         sg = rng.uniform(2.0, max(3.0, min(h, w) / 12.0))
         amp = rng.uniform(-3.0, 3.0)
-        f += amp * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg * sg))
+        # the blob within 9 sigma of its centre (beyond, each term is below e^-40.5 of amp)
+        y0, y1 = max(0, int(cy - 9 * sg)), min(h, int(cy + 9 * sg) + 2)
+        x0, x1 = max(0, int(cx - 9 * sg)), min(w, int(cx + 9 * sg) + 2)
+        if y0 < y1 and x0 < x1:
+            gy = np.exp(-((np.arange(y0, y1, dtype=np.float64) - cy) ** 2) / (2 * sg * sg))
+            gx = np.exp(-((np.arange(x0, x1, dtype=np.float64) - cx) ** 2) / (2 * sg * sg))
+            f[y0:y1, x0:x1] += amp * np.outer(gy, gx)
     f -= f.min()
     f /= max(f.max(), 1e-12)
     return f
