@@ -42,7 +42,8 @@ def kernel_bytes(plan, S, canvas_px, nx, ny, ch):
     out = {
         "warp_stack": 20 * S + 20 * canvas_px,
         "paste_stack": 20 * S + 20 * canvas_px,
-        "pack_rowfft": 2 * 20 * canvas_px + (PA + PB) * ch * nx * 8,
+        # a translation plan's pack reads the images in place (no canvases)
+        "pack_rowfft": 2 * 20 * (S if TB else canvas_px) + (PA + PB) * ch * nx * 8,
         "colfft_fwd": (PA + PB) * (ch * nx * 8 + N * 8),
         "cmul": (PA + PB) * N * 8 + Q * N * 8,
         "colfft_inv": 2 * Q * N * 8,

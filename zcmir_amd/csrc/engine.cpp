@@ -602,11 +602,15 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
   double Hi[9];
   warp_inverse(Hc, Hi);
   const float2* y_tw = twiddles(ny);
-  if (dev_hi)
-    LAUNCH("warp_stack", launch_warp_stack_dev(A.stack.as<float>(), A.w, A.h, dev_hi, canvas_a.as<float>(), W, Hh, st));
-  else
-    LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
-  LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
+  // a translation plan's canvases are the images moved by integers: the pack reads the images in
+  // place, so neither canvas is written
+  if (!trans) {
+    if (dev_hi)
+      LAUNCH("warp_stack", launch_warp_stack_dev(A.stack.as<float>(), A.w, A.h, dev_hi, canvas_a.as<float>(), W, Hh, st));
+    else
+      LAUNCH("warp_stack", launch_warp_stack(A.stack.as<float>(), A.w, A.h, Hi, canvas_a.as<float>(), W, Hh, st));
+    LAUNCH("paste_stack", launch_paste_stack(B.stack.as<float>(), B.w, B.h, m.fr.ox, m.fr.oy, canvas_b.as<float>(), W, Hh, st));
+  }
   bool ok = true;
   // A rectangle plan's tables depend only on canvas A: unless launches are profiled (one event
   // chain on one stream), they are built on the second stream while the FFT stages run.
@@ -649,8 +653,15 @@ MapOut Engine::shift_map_sides(Side& A, Side& B, const Mat3& Hp, bool exact, boo
     cells_buf.ensure((size_t)cw * ch * 5 * 4);
     cells = cells_buf.as<float>();
   }
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st, cells));
-  LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st, cells));
+  if (trans) {
+    LAUNCH("pack_rowfft", ok &= launch_pack_rowfft_shifted(plan, 0, A.stack.as<float>(), A.w, A.h, (int)Hc[2], (int)Hc[5],
+                                                           cw, ch, nx, ny, spa, twiddles(nx), st));
+    LAUNCH("pack_rowfft", ok &= launch_pack_rowfft_shifted(plan, 1, B.stack.as<float>(), B.w, B.h, -m.fr.ox, -m.fr.oy,
+                                                           cw, ch, nx, ny, spb, twiddles(nx), st));
+  } else {
+    LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 0, canvas_a.as<float>(), W, Hh, cw, ch, nx, ny, spa, twiddles(nx), st, cells));
+    LAUNCH("pack_rowfft", ok &= launch_pack_rowfft(plan, 1, canvas_b.as<float>(), W, Hh, cw, ch, nx, ny, spb, twiddles(nx), st, cells));
+  }
   LAUNCH("colfft_fwd", ok &= launch_colfft(spa, pi.n_cplx + pi.b_n_cplx, nx, ny, ch, false, y_tw, st, col_lines(ny)));
   LAUNCH("cmul", launch_cmul(plan, spa, spb, work.as<float2>(), nx, ny, st));
   LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out, nx, ny, ny, true, y_tw, st, col_lines(ny)));

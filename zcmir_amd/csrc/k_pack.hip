@@ -355,12 +355,22 @@ bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, i
   r.kx = (int)fminf(fmaxf(ceilf(r.fx - 1e-4f), 1.0f), 8.0f);
   r.ky = (int)fminf(fmaxf(ceilf(r.fy - 1e-4f), 1.0f), 8.0f);
   r.identity = (cw == W && ch == H) ? 1 : 0;
+  r.sx = r.sy = 0;
   if (!r.identity && cells && cw > 0 && ch > 0) {
     launch_resample(canvas, r, cells, st);
     canvas = cells;
-    r = Resample{cw, ch, cw, ch, 1, 1, 1, 1.0f, 1.0f};
+    r = Resample{cw, ch, cw, ch, 1, 1, 1, 1.0f, 1.0f, 0, 0};
   }
 #define CALL(LL) pack_rowfft_L<LL>(plan, side, canvas, r, ny, spec, tw_x, st)
+  ZC_POW2_SWITCH(nx, CALL)
+#undef CALL
+  return true;
+}
+
+bool launch_pack_rowfft_shifted(int plan, int side, const float* stack, int sw, int sh, int sx, int sy, int cw,
+                                int ch, int nx, int ny, float2* spec, const float2* tw_x, hipStream_t st) {
+  const Resample r{sw, sh, cw, ch, 1, 1, 2, 1.0f, 1.0f, sx, sy};
+#define CALL(LL) pack_rowfft_L<LL>(plan, side, stack, r, ny, spec, tw_x, st)
   ZC_POW2_SWITCH(nx, CALL)
 #undef CALL
   return true;

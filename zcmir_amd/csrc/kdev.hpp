@@ -55,9 +55,13 @@ __device__ __forceinline__ constexpr int tri_k(int i, int j) {
 
 
 // ── maps: pack + row FFT ──────────────────────────────────────────────────────────────────
+// identity: 1 the cells are the canvas pixels; 2 they are an image's pixels moved by (sx, sy) (a
+// stack of W × H planes; cells outside the image are 0): an integer translation onto the canvas
+// read without a canvas copy; 0 the k×k box resample
 struct Resample {
   int W, H, cw, ch, kx, ky, identity;
   float fx, fy;
+  int sx, sy;
 };
 
 __device__ __forceinline__ float sample_plane(const float* __restrict__ s, int W, int H, float px, float py) {
@@ -77,6 +81,13 @@ __device__ __forceinline__ float sample_plane(const float* __restrict__ s, int W
 __device__ __forceinline__ void cell_values(const float* __restrict__ canvas, const Resample& r, int x, int y,
                                             float* s) {
   const int64_t n = (int64_t)r.W * r.H;
+  if (r.identity == 2) {
+    const int ix = x - r.sx, iy = y - r.sy;
+    const bool in = ix >= 0 && iy >= 0 && ix < r.W && iy < r.H;
+#pragma unroll
+    for (int p = 0; p < 5; ++p) s[p] = in ? canvas[p * n + (int64_t)iy * r.W + ix] : 0.0f;
+    return;
+  }
   if (r.identity) {
 #pragma unroll
     for (int p = 0; p < 5; ++p) s[p] = canvas[p * n + (int64_t)y * r.W + x];

@@ -55,6 +55,10 @@ void launch_paste_stack(const float* stack, int sw, int sh, int ox, int oy, floa
 // (5 · cw · ch floats, optional): where a resampled grid's cell values are gathered first.
 bool launch_pack_rowfft(int plan, int side, const float* canvas, int W, int H, int cw, int ch, int nx, int ny,
                         float2* spec, const float2* tw_x, hipStream_t st, float* cells = nullptr);
+// The same for one cell per canvas pixel when the canvas holds an image (a stack of sw × sh
+// planes) moved by (sx, sy) and nothing else: the image is read in place, no canvas is written.
+bool launch_pack_rowfft_shifted(int plan, int side, const float* stack, int sw, int sh, int sx, int sy, int cw,
+                                int ch, int nx, int ny, float2* spec, const float2* tw_x, hipStream_t st);
 // column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
 // lines: columns per workgroup (0: the default for the length)
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
