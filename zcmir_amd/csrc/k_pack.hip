@@ -139,15 +139,14 @@ __device__ __forceinline__ float2 unpack2(float2 z, float2 zm, bool odd) {
   return odd ? make_float2(z.y + zm.y, zm.x - z.x) : make_float2(z.x + zm.x, z.y - zm.y);
 }
 
+// Batch entries blockIdx.y · nb .. + nb (at most `batch`) in turn per thread: the fixed spectra B,
+// shared by every entry, are read once per thread and kept in registers.
 template <int PID>
 __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, const float2* __restrict__ B,
-                                              float2* work, int nx, int ny, float scale) {
+                                              float2* work, int nx, int ny, float scale, int nb, int batch) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr int NC = P.n_cplx, NCB = P.b_n_cplx;
   const int64_t N = (int64_t)nx * ny;
-  // blockIdx.y: the batch entry (its NC spectra in A, its n_out outputs in work; B is shared)
-  A += (int64_t)blockIdx.y * NC * N;
-  work += (int64_t)blockIdx.y * P.n_out * N;
   const int64_t i = blockIdx.x * 256 + threadIdx.x;
   const int64_t half_rows = ny / 2 + 1;
   if (i >= half_rows * nx) return;
@@ -156,35 +155,42 @@ __global__ __launch_bounds__(256) void k_cmul(const float2* __restrict__ A, cons
   if ((ky == 0 || 2 * ky == ny) && kx > nx / 2) return;   // its mirror bin covers it
   const int64_t k = (int64_t)ky * nx + kx, km = (int64_t)my * nx + mx;
   const bool self = k == km;
-  float2 za[NC], zam[NC], zb[NCB], zbm[NCB];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    za[c] = A[c * N + k];
-    zam[c] = A[c * N + km];
-  }
+  float2 zb[NCB], zbm[NCB];
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
     zb[c] = B[c * N + k];
     zbm[c] = B[c * N + km];
   }
   const float sc = 0.25f * scale;
+  const int e1 = min(batch, ((int)blockIdx.y + 1) * nb);
+  for (int e = (int)blockIdx.y * nb; e < e1; ++e) {
+    // entry e: its NC spectra in A, its n_out outputs in work
+    const float2* Ae = A + (int64_t)e * NC * N;
+    float2* we = work + (int64_t)e * P.n_out * N;
+    float2 za[NC], zam[NC];
 #pragma unroll
-  for (int q = 0; q < P.n_out; ++q) {
-    const int c1 = 2 * q, c2 = 2 * q + 1;
-    const int a1 = P.ca[c1], b1 = P.cb[c1];
-    float2 xa = unpack2(za[a1 >> 1], zam[a1 >> 1], a1 & 1), xb = unpack2(zb[b1 >> 1], zbm[b1 >> 1], b1 & 1);
-    float2 ya = unpack2(zam[a1 >> 1], za[a1 >> 1], a1 & 1), yb = unpack2(zbm[b1 >> 1], zb[b1 >> 1], b1 & 1);
-    float2 p = cmulc(xa, xb), pm = cmulc(ya, yb);
-    if (c2 < P.n_corr) {
-      const int a2 = P.ca[c2], b2 = P.cb[c2];
-      const float2 xa2 = unpack2(za[a2 >> 1], zam[a2 >> 1], a2 & 1), xb2 = unpack2(zb[b2 >> 1], zbm[b2 >> 1], b2 & 1);
-      const float2 ya2 = unpack2(zam[a2 >> 1], za[a2 >> 1], a2 & 1), yb2 = unpack2(zbm[b2 >> 1], zb[b2 >> 1], b2 & 1);
-      const float2 p2 = cmulc(xa2, xb2), pm2 = cmulc(ya2, yb2);
-      p = make_float2(p.x - p2.y, p.y + p2.x);       // p + i p2
-      pm = make_float2(pm.x - pm2.y, pm.y + pm2.x);
+    for (int c = 0; c < NC; ++c) {
+      za[c] = Ae[c * N + k];
+      zam[c] = Ae[c * N + km];
     }
-    work[q * N + k] = make_float2(p.x * sc, p.y * sc);
-    if (!self) work[q * N + km] = make_float2(pm.x * sc, pm.y * sc);
+#pragma unroll
+    for (int q = 0; q < P.n_out; ++q) {
+      const int c1 = 2 * q, c2 = 2 * q + 1;
+      const int a1 = P.ca[c1], b1 = P.cb[c1];
+      float2 xa = unpack2(za[a1 >> 1], zam[a1 >> 1], a1 & 1), xb = unpack2(zb[b1 >> 1], zbm[b1 >> 1], b1 & 1);
+      float2 ya = unpack2(zam[a1 >> 1], za[a1 >> 1], a1 & 1), yb = unpack2(zbm[b1 >> 1], zb[b1 >> 1], b1 & 1);
+      float2 p = cmulc(xa, xb), pm = cmulc(ya, yb);
+      if (c2 < P.n_corr) {
+        const int a2 = P.ca[c2], b2 = P.cb[c2];
+        const float2 xa2 = unpack2(za[a2 >> 1], zam[a2 >> 1], a2 & 1), xb2 = unpack2(zb[b2 >> 1], zbm[b2 >> 1], b2 & 1);
+        const float2 ya2 = unpack2(zam[a2 >> 1], za[a2 >> 1], a2 & 1), yb2 = unpack2(zbm[b2 >> 1], zb[b2 >> 1], b2 & 1);
+        const float2 p2 = cmulc(xa2, xb2), pm2 = cmulc(ya2, yb2);
+        p = make_float2(p.x - p2.y, p.y + p2.x);       // p + i p2
+        pm = make_float2(pm.x - pm2.y, pm.y + pm2.x);
+      }
+      we[q * N + k] = make_float2(p.x * sc, p.y * sc);
+      if (!self) we[q * N + km] = make_float2(pm.x * sc, pm.y * sc);
+    }
   }
 }
 
@@ -406,20 +412,25 @@ void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* w
                  int batch) {
   const int64_t cnt = (int64_t)(ny / 2 + 1) * nx;
   const float scale = 1.0f / ((float)nx * (float)ny);
-  const dim3 g(blocks_for(cnt, 256), batch);
+  // entries per thread: as many as leave at least 4096 workgroups (one per entry when the bins
+  // alone give fewer)
+  const unsigned gx = blocks_for(cnt, 256);
+  const int rows = (int)std::min<int64_t>(batch, std::max<int64_t>(1, (4096 + gx - 1) / gx));
+  const int nb = (batch + rows - 1) / rows;
+  const dim3 g(gx, (unsigned)((batch + nb - 1) / nb));
   switch (plan) {
-    case 0: k_cmul<0><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 1: k_cmul<1><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 2: k_cmul<2><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 3: k_cmul<3><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 4: k_cmul<4><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 5: k_cmul<5><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 6: k_cmul<6><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 7: k_cmul<7><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 8: k_cmul<8><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 9: k_cmul<9><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    case 10: k_cmul<10><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
-    default: k_cmul<11><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale); break;
+    case 0: k_cmul<0><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 1: k_cmul<1><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 2: k_cmul<2><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 3: k_cmul<3><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 4: k_cmul<4><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 5: k_cmul<5><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 6: k_cmul<6><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 7: k_cmul<7><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 8: k_cmul<8><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 9: k_cmul<9><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    case 10: k_cmul<10><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
+    default: k_cmul<11><<<g, 256, 0, st>>>(spec_a, spec_b, work, nx, ny, scale, nb, batch); break;
   }
 }
 

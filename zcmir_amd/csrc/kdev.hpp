@@ -141,8 +141,10 @@ struct RowT {
 template <int L, int V>
 struct RowX {
   static constexpr bool WIDE = V == 1 || (V == 0 && L >= 2048);
-  // lines of 2^a·3^b points: 48, 12 or 24 points per thread in place of 32, 8 or 16
-  static constexpr int M = L % 3 == 0 ? (WIDE ? 48 : V == 2 ? 12 : 24)
+  // lines of 2^a·3^b points: 48, 12 or 24 points per thread in place of 32, 8 or 16; a line of
+  // 9 · 2^k points takes 12 (2304: 192 lanes and radices 12·12·16, where 48 points left a
+  // 96-lane workgroup of two lines)
+  static constexpr int M = L % 3 == 0 ? (L % 9 == 0 ? 12 : WIDE ? 48 : V == 2 ? 12 : 24)
                                       : (WIDE ? 32 : (V == 2 || (V == 0 && L == 256)) ? 8 : 16);
   static constexpr int TWO = (V == 4 || (V == 0 && L == 1024)) ? 2 : 1;
   static constexpr int MIN_LINES = (64 * M + L - 1) / L;
