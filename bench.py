@@ -403,23 +403,30 @@ def bench_prior(a, np, torch, zcmir_amd):
     pixel)."""
     from zcmir_amd import prior, synth
 
-    if not prior.WEIGHTS.exists():
-        return {"error": f"{prior.WEIGHTS.name} missing"}
-    net = prior.Net.load()
-    rng = np.random.default_rng(20251)
-    n_eval, n_cal = a.prior_pairs, 8
-    size = 1024
-    tex = torch.tensor(np.stack([synth._texture(size, size, rng) for _ in range(n_eval + n_cal)]).astype(np.float32),
-                       device="cuda")
-    Hs, th, sg, rm = prior.random_poses(rng, n_eval + n_cal, size)
-    mv, fx = prior.make_pairs(tex, np.arange(n_eval + n_cal), Hs, rm)
+    weights = Path(a.prior_weights) if a.prior_weights else prior.WEIGHTS
+    if not weights.exists():
+        return {"error": f"{weights.name} missing"}
+    net = prior.Net.load(weights)
+    n_eval, n_cal, size = a.prior_pairs, a.prior_cal, 1024
+
+    def pairs(seed, n):
+        rng = np.random.default_rng(seed)
+        tex = torch.tensor(np.stack([synth._texture(size, size, rng) for _ in range(n)]).astype(np.float32),
+                           device="cuda")
+        Hs, th, sg, rm = prior.random_poses(rng, n, size)
+        mv, fx = prior.make_pairs(tex, np.arange(n), Hs, rm)
+        return mv, fx, Hs, th, sg
+
+    # held-out evaluation pairs, and calibration pairs of their own draw
+    mv, fx, Hs, th, sg = pairs(20251, n_eval)
+    cmv, cfx, cHs, cth, csg = pairs(20252, n_cal)
     torch.cuda.synchronize()
     reg = zcmir_amd.Registrar(half=False, sw_full=True, **SWEEP_GRID)
-    # the fallback threshold, from held-out pairs (the last n_cal) at their true cells
-    thr = prior.calibrate(reg, net, mv[n_eval:], fx[n_eval:], Hs[n_eval:], th[n_eval:], sg[n_eval:])
+    # the fallback threshold: the 5th percentile of the calibration pairs' check scores at their true cells
+    thr = prior.calibrate(reg, net, cmv, cfx, cHs, cth, csg)
     # warm both paths once (first-use allocations and twiddle tables)
     reg.set_images(mv[0].cpu().numpy(), fx[0].cpu().numpy(), preprocess=False)
-    prior.learned_search(reg, net, mv[0], fx[0], threshold=thr)
+    prior.learned_search(reg, net, mv[0], fx[0], threshold=thr, margin=a.prior_margin)
     rows = []
     for i in range(n_eval):
         m_np, f_np = mv[i].cpu().numpy(), fx[i].cpu().numpy()
@@ -432,14 +439,19 @@ def bench_prior(a, np, torch, zcmir_amd):
         reg.pose = np.eye(3)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lrn, info = prior.learned_search(reg, net, mv[i], fx[i], threshold=thr)
+        lrn, info = prior.learned_search(reg, net, mv[i], fx[i], threshold=thr, margin=a.prior_margin)
         torch.cuda.synchronize()
         t_lrn = time.perf_counter() - t0
         d, dth = _pose_close(lrn.H, full.H, size, size)
+        # the label-grid cell of the full search's pose (its rotation and scale)
+        A = np.asarray(full.H, float)[:2, :2]
+        full_cell = prior.truth_cell(float(np.degrees(np.arctan2(A[1, 0] - A[0, 1], A[0, 0] + A[1, 1]))),
+                                     float(np.sqrt(abs(np.linalg.det(A)))))
         rows.append({"t_full_s": round(t_full, 4), "t_learned_s": round(t_lrn, 4), "pose_dist_px": round(d, 4),
                      "rot_diff_deg": round(dth, 4), "fell_back": bool(info["fell_back"]),
                      "check_score": round(info["check_score"], 4),
                      "truth_cell_proposed": prior.truth_cell(th[i], sg[i]) in info["cells"],
+                     "full_cell_proposed": full_cell in info["cells"],
                      "truth_dist_px": round(_pose_close(full.H, Hs[i], size, size)[0], 3),
                      "learned_truth_dist_px": round(_pose_close(lrn.H, Hs[i], size, size)[0], 3)})
     reg.close()
@@ -449,7 +461,7 @@ def bench_prior(a, np, torch, zcmir_amd):
     return {"metric": "time-to-pose learned vs full sweep", "value": round(tf / tl, 2), "unit": "x",
             "time_to_pose_full_s": round(tf, 4), "time_to_pose_learned_s": round(tl, 4),
             "same_pose_rate": float(np.mean(same)), "fallbacks": int(sum(r["fell_back"] for r in rows)),
-            "threshold": thr, "pairs": rows,
+            "threshold": thr, "calibration_pairs": n_cal, "margin_rotations": a.prior_margin, "pairs": rows,
             "network": {"layers": prior.LAYERS, "input": [prior.T, prior.R, prior.C],
                         "trained_pairs": int(net.meta.get("n_pairs", 0))}}
 
@@ -468,6 +480,9 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=10, help="timed sweeps on the 128² canvas (0: skip)")
     ap.add_argument("--sweep-full-steps", type=int, default=1, help="timed full-resolution sweeps (0: skip)")
     ap.add_argument("--prior-pairs", type=int, default=24, help="time-to-pose pairs, learned vs full (0: skip)")
+    ap.add_argument("--prior-cal", type=int, default=32, help="calibration pairs of the fallback threshold")
+    ap.add_argument("--prior-margin", type=int, default=1, help="fine rotations swept beyond each proposed class")
+    ap.add_argument("--prior-weights", help="network weights (default: the package's prior_weights.npz)")
     ap.add_argument("--refine-runs", type=int, default=3, help="timed climbs on the 1024² pair (0: skip)")
     ap.add_argument("--rs-runs", type=int, default=10, help="timed roto-scale maps on a 1024² pair (0: skip)")
     ap.add_argument("--sweep-batch", type=int, default=0, help="sweep candidates per batch (0: 256, the default)")

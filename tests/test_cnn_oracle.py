@@ -74,6 +74,37 @@ def test_lp_features_rotation_is_a_shift():
     assert int(np.argmin(err)) == 24
 
 
+def test_xcorr_channel_is_the_direct_correlation():
+    """The input's third channel against the direct double sum c[t, d + R/2] = Σ_{t', r}
+    a[(t' + t) mod T, r + d] b[t', r] / (T (R − |d|)) (then standardized), and a map shifted by
+    k angle bins and j radius bins against itself peaks at lag (k, j)."""
+    import torch
+
+    from zcmir_amd import prior
+
+    rng = np.random.default_rng(11)
+    T, R = 12, 10
+    a, b = rng.standard_normal((T, R)), rng.standard_normal((T, R))
+    got = prior.xcorr_channel(torch.tensor(a[None]), torch.tensor(b[None]))[0].numpy()
+    ref = np.zeros((T, R))
+    for t in range(T):
+        for d in range(-(R // 2), R - R // 2):
+            s = 0.0
+            for t2 in range(T):
+                for r in range(R):
+                    if 0 <= r + d < R:
+                        s += a[(t2 + t) % T, r + d] * b[t2, r]
+            ref[t, d + R // 2] = s / (T * (R - abs(d)))
+    ref = (ref - ref.mean()) / ref.std(ddof=1)
+    assert np.abs(got - ref).max() < 1e-9
+    big = rng.standard_normal((48, 64))
+    moved = np.roll(big, 5, axis=0)
+    moved[:, 3:] = moved[:, :-3].copy()
+    c = prior.xcorr_channel(torch.tensor(moved[None]), torch.tensor(big[None]))[0].numpy()
+    t, d = np.unravel_index(np.argmax(c), c.shape)
+    assert (t, d - 32) == (5, 3)
+
+
 def test_fine_cells_cover_the_coarse_rotation():
     from zcmir_amd import prior
 

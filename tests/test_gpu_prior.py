@@ -59,7 +59,7 @@ def test_dense_layer_mfma(env):
 
 
 @pytest.mark.parametrize("n,h,w,cin,cout,stride", [(2, 8, 12, 2, 32, 1), (3, 12, 16, 32, 64, 2), (2, 6, 10, 5, 7, 2),
-                                                   (1, 48, 64, 2, 32, 1)])
+                                                   (1, 48, 64, 2, 32, 1), (2, 48, 64, 3, 32, 1)])
 def test_conv_layer_mfma(env, n, h, w, cin, cout, stride):
     lib, torch = env
     rng = np.random.default_rng(cin * 100 + cout)

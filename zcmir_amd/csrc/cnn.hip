@@ -487,7 +487,8 @@ int32_t zc_cnn_adam(float* p, const float* g, float* m, float* v, int64_t n, flo
 
 // Input features of `count` gray images (w × h each, device): S × S downsample → rank → Sobel
 // magnitude × Hann → 2-D FFT → log(1 + |F|) on T angles × R radii → channel `ch` of the NHWC
-// [count][T][R][C] output, normalized per map. work: count · S² · 12 bytes of device scratch.
+// [count][T][R][C] output. Writing the last channel (ch = C − 1) standardizes every channel of
+// the output per map (zero mean, unit deviation). work: count · S² · 12 bytes of device scratch.
 int32_t zc_lp_features(const float* imgs, int count, int w, int h, int S, int T, int R, int C, int ch, float r0,
                        float r1, float* out, void* work, void* st) {
   if (S < 32 || S > 4096 || (S & (S - 1)) || ch < 0 || ch >= C) return -1;

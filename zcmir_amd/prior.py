@@ -4,8 +4,10 @@ Each image of a pair becomes a log-polar map of its Fourier magnitude (zc_lp_fea
 downsample, rank, Sobel magnitude, FFT, 48 angles over [0, π) × 64 log radii over 4 .. 128
 cycles): translation drops out, a rotation of the image is a periodic shift along the angle
 axis and a scale change a shift along the radius axis. The two maps (moving, fixed) are the two
-channels of the network's input; six 3×3 convolutions (32 – 64 channels, ReLU, three of stride
-2) and a dense head give logits over 24 rotations (15° steps) × 9 scales (0.8 – 1.25, log-spaced).
+channels of the network's input, with their cross-correlation over angle and log-radius lags as a
+third (a rotation and a scale change move its peak); six 3×3 convolutions (32 – 64 channels,
+ReLU, three of stride 2) and a dense head give logits over 24 rotations (15° steps) × 9 scales
+(0.8 – 1.25, log-spaced).
 The magnitude spectrum cannot tell θ from θ + 180°, so the network spreads its belief over both;
 the SMI scores of the proposed cells decide.
 
@@ -23,8 +25,8 @@ import numpy as np
 
 from . import _native
 
-LAYERS = [(2, 32, 1), (32, 32, 2), (32, 64, 1), (64, 64, 2), (64, 64, 1), (64, 64, 2)]
-T, R, C = 48, 64, 2            # angles, log radii, channels of the input map
+LAYERS = [(3, 32, 1), (32, 32, 2), (32, 64, 1), (64, 64, 2), (64, 64, 1), (64, 64, 2)]
+T, R, C = 48, 64, 3            # angles, log radii, channels of the input map
 S = 256                        # the images' downsampled side
 R0, R1 = 4.0, 128.0            # radius range (cycles per 256 px)
 N_TH, N_SG = 24, 9             # label grid: 24 rotations × 9 scales
@@ -158,18 +160,42 @@ class Net:
         return loss.mean()
 
 
+def xcorr_channel(a, b):
+    """The two log-polar maps' cross-correlation (a, b: [n, T, R] tensors): circular over the T
+    angle lags, linear over the log-radius lags d = −R/2 .. R/2 − 1 (each normalized by its
+    overlap T (R − |d|)), standardized per pair; [n, T, R] with lag d in column d + R/2. A
+    rotation of the pair is a shift of its peak along the angle lags, a scale change one along the
+    radius lags (the Fourier–Mellin correlation of the two spectra)."""
+    import torch
+
+    n, t, r = a.shape
+    fa = torch.fft.fft2(torch.nn.functional.pad(a, (0, r)))
+    fb = torch.fft.fft2(torch.nn.functional.pad(b, (0, r)))
+    x = torch.fft.ifft2(fa * torch.conj(fb)).real                     # [n, T, 2R], lag d at d mod 2R
+    d = torch.arange(-(r // 2), r - r // 2, device=a.device)
+    c = x[:, :, d % (2 * r)] / (t * (r - d.abs()).to(x.dtype))
+    c = c - c.mean(dim=(1, 2), keepdim=True)
+    return c / c.flatten(1).std(dim=1).clamp_min(1e-12)[:, None, None]
+
+
 def features(moving, fixed):
-    """Network input [n, T, R, 2] from CUDA tensors moving, fixed [n, h, w] (gray, [0, 1])."""
+    """Network input [n, T, R, 3] from CUDA tensors moving, fixed [n, h, w] (gray, [0, 1]): the
+    two log-polar maps (zc_lp_features) and their cross-correlation (xcorr_channel)."""
     import torch
 
     n, h, w = moving.shape
-    out = torch.empty((n, T, R, C), dtype=torch.float32, device="cuda")
+    # the two maps as a 2-channel tensor: zc_lp_features standardizes every channel once the
+    # last one is written
+    lp = torch.empty((n, T, R, 2), dtype=torch.float32, device="cuda")
     work = torch.empty(n * S * S * 3, dtype=torch.float32, device="cuda")
     lib, st = _lib(), _stream()
     for ch, img in ((0, moving), (1, fixed)):
         hh, ww = img.shape[1], img.shape[2]
-        _ok(lib.zc_lp_features(_p(img.contiguous()), n, ww, hh, S, T, R, C, ch, R0, R1, _p(out), _p(work), st),
+        _ok(lib.zc_lp_features(_p(img.contiguous()), n, ww, hh, S, T, R, 2, ch, R0, R1, _p(lp), _p(work), st),
             "lp_features")
+    out = torch.empty((n, T, R, C), dtype=torch.float32, device="cuda")
+    out[..., :2] = lp
+    out[..., 2] = xcorr_channel(lp[..., 0], lp[..., 1])
     return out
 
 
@@ -373,9 +399,9 @@ def make_pairs(textures, idx, Hs, remaps):
     return moving, fixed
 
 
-def sweep_labels(reg, moving, fixed, grid=64):
-    """The sweep's argmax cell (24 × 9 grid) for each pair: the label of the pair, and the cell's
-    best SMI score divided by the fixed image's area."""
+def sweep_labels(reg, moving, fixed, grid=128):
+    """The sweep's argmax cell (24 × 9 grid, on a grid² canvas) for each pair: the label of the
+    pair, and the cell's best SMI score divided by the fixed image's area."""
     import torch
 
     torch.cuda.synchronize()
@@ -393,52 +419,113 @@ def sweep_labels(reg, moving, fixed, grid=64):
     return np.array(labels, np.int32), np.array(best_norm)
 
 
-def train(n_pairs=16000, n_val=1000, size=512, epochs=12, batch=128, lr=1e-3, seed=1, log=print, out=WEIGHTS):
-    """Generate and label pairs on the GPU, train the network, save WEIGHTS. Returns metrics."""
+CHUNK = 500   # pairs generated (and their textures drawn) together
+
+
+def _pair_chunk(seed, size, s0, s1, Hs, rm):
+    """Pairs s0 .. s1 of a training set (s0 a multiple of CHUNK): each its own texture."""
+    textures = gpu_textures(s1 - s0, size, seed * 1000003 + s0)
+    return make_pairs(textures, np.arange(s1 - s0), Hs[s0:s1], rm[s0:s1])
+
+
+def label_pairs(total, size=1024, seed=1, start=0, stop=None, grid=128, log=print):
+    """Sweep labels of pairs [start, stop) of a training set of `total` pairs (start a multiple
+    of CHUNK). Returns a dict of arrays: labels, best-score norms, whether the label is the nearest
+    true cell, and the range."""
+    import time
+
+    import zcmir_amd
+
+    stop = total if stop is None else stop
+    if start % CHUNK:
+        raise ValueError(f"start must be a multiple of {CHUNK}")
+    rng = np.random.default_rng(seed)
+    Hs, th, sg, rm = random_poses(rng, total, size)
+    reg = zcmir_amd.Registrar(half=False)
+    labels, norms, agree = [], [], []
+    t0 = time.time()
+    for s0 in range(start, stop, CHUNK):
+        s1 = min(stop, s0 + CHUNK)
+        mv, fx = _pair_chunk(seed, size, s0, s1, Hs, rm)
+        lab, bn = sweep_labels(reg, mv, fx, grid)
+        labels.append(lab)
+        norms.append(bn)
+        agree += [truth_cell(th[i], sg[i]) == (lab[i - s0] // N_SG, lab[i - s0] % N_SG) for i in range(s0, s1)]
+        log(f"labelled {s1 - start}/{stop - start}  {time.time() - t0:.1f} s  label = nearest true cell: "
+            f"{np.mean(agree):.3f}")
+    reg.close()
+    return {"labels": np.concatenate(labels), "norms": np.concatenate(norms), "agree": np.array(agree),
+            "range": np.array([start, stop]), "total": np.array(total), "size": np.array(size),
+            "seed": np.array(seed), "grid": np.array(grid)}
+
+
+def train(n_pairs=16000, n_val=1000, size=512, epochs=12, batch=128, lr=1e-3, seed=1, log=print, out=WEIGHTS,
+          labels=None, grid=128, budget_s=None):
+    """Generate pairs on the GPU, label them by the sweep (or take `labels`: label_pairs' output
+    covering every pair), train the network, save WEIGHTS. Returns metrics. With `budget_s`, the
+    epochs are cut to those that fit the seconds left after the features (timed on the first 16
+    steps, which count towards the first epoch)."""
     import time
 
     import torch
 
-    import zcmir_amd
-
-    rng = np.random.default_rng(seed)
     t0 = time.time()
-    reg = zcmir_amd.Registrar(half=False)
     total = n_pairs + n_val
+    if labels is None:
+        labels = label_pairs(total, size, seed, 0, total, grid, log)
+    if len(labels["labels"]) != total:
+        raise ValueError(f"labels for {len(labels['labels'])} pairs, the set has {total}")
+    rng = np.random.default_rng(seed)
     Hs, th, sg, rm = random_poses(rng, total, size)
-    X, Y, agree, norms = [], [], [], []
-    for s0 in range(0, total, 500):
-        s1 = min(total, s0 + 500)
-        # every pair its own texture (validation pairs included: held-out textures)
-        textures = gpu_textures(s1 - s0, size, seed * 1000003 + s0)
-        mv, fx = make_pairs(textures, np.arange(s1 - s0), Hs[s0:s1], rm[s0:s1])
-        del textures
-        lab, bn = sweep_labels(reg, mv, fx)
+    X = []
+    for s0 in range(0, total, CHUNK):
+        s1 = min(total, s0 + CHUNK)
+        mv, fx = _pair_chunk(seed, size, s0, s1, Hs, rm)
         X.append(features(mv, fx))
-        Y.append(lab)
-        norms.append(bn)
-        agree += [truth_cell(th[i], sg[i]) == (lab[i - s0] // N_SG, lab[i - s0] % N_SG) for i in range(s0, s1)]
-        log(f"pairs {s1}/{total}  {time.time() - t0:.1f} s  label = nearest true cell: {np.mean(agree):.3f}")
-    reg.close()
+        del mv, fx
+        if (s0 // CHUNK) % 10 == 0:
+            log(f"features {s1}/{total}  {time.time() - t0:.1f} s")
     X = torch.cat(X)
-    Y = torch.tensor(np.concatenate(Y), device="cuda")
-    norms = np.concatenate(norms)
+    Y = torch.tensor(labels["labels"], device="cuda")
+    norms = labels["norms"]
+    agree = labels["agree"]
     net = Net(seed)
     xtr, ytr, xva, yva = X[:n_pairs], Y[:n_pairs], X[n_pairs:], Y[n_pairs:]
     steps = 0
-    for ep in range(epochs):
+    per_epoch = n_pairs // batch
+    ep = 0
+    best = None
+    while ep < epochs:
         perm = torch.randperm(n_pairs, device="cuda")
         run = 0.0
         for b0 in range(0, n_pairs - batch + 1, batch):
             sel = perm[b0:b0 + batch]
-            lr_t = lr * 0.5 * (1 + math.cos(math.pi * steps / (epochs * (n_pairs // batch))))
+            lr_t = lr * 0.5 * (1 + math.cos(math.pi * steps / (epochs * per_epoch)))
+            if budget_s is not None and steps == 16:
+                torch.cuda.synchronize()
+                t_step = (time.time() - t_fit) / 16
+                fit = int((budget_s - (time.time() - t0)) / (t_step * per_epoch * 1.05))
+                if fit < epochs:
+                    epochs = max(1, fit)
+                    log(f"{t_step * 1e3:.1f} ms per step: {epochs} epochs fit the budget")
+            if steps == 0:
+                t_fit = time.time()
             run += float(net.step(xtr[sel], ytr[sel], lr_t))
             steps += 1
         acc = evaluate(net, xva, yva)
-        log(f"epoch {ep + 1}: loss {run / (n_pairs // batch):.3f}  val top1 {acc['top1']:.3f} top8 {acc['top8']:.3f} "
+        log(f"epoch {ep + 1}: loss {run / per_epoch:.3f}  val top1 {acc['top1']:.3f} top8 {acc['top8']:.3f} "
             f"top8±1θ {acc['top8_near']:.3f}  {time.time() - t0:.1f} s")
+        ep += 1
+        # early stopping on the held-out pairs: keep the weights of the best epoch
+        if best is None or acc["top8_near"] > best[0]:
+            best = (acc["top8_near"], ep, [t.detach().clone() for t in net.p])
+    with torch.no_grad():
+        for t, b in zip(net.p, best[2]):
+            t.copy_(b)
+    log(f"kept epoch {best[1]} (val top8±1θ {best[0]:.3f})")
     thr = float(np.quantile(norms[n_pairs:], 0.05))
-    net.save(out, threshold=thr, label_agreement=np.mean(agree), n_pairs=n_pairs, size=size)
+    net.save(out, threshold=thr, label_agreement=float(np.mean(agree)), n_pairs=n_pairs, size=size,
+             label_grid=int(np.asarray(labels.get("grid", grid))), epochs=epochs, kept_epoch=best[1])
     return dict(evaluate(net, xva, yva), threshold=thr, label_agreement=float(np.mean(agree)),
                 seconds=time.time() - t0)
 
@@ -474,9 +561,28 @@ def main():
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--grid", type=int, default=128, help="the labelling sweep's canvas cells per side")
     ap.add_argument("--out", default=str(WEIGHTS))
+    ap.add_argument("--label-range", type=int, nargs=2, metavar=("START", "STOP"),
+                    help="only label pairs [START, STOP) of the set and save them to --labels-out")
+    ap.add_argument("--labels-out", default="labels.npz")
+    ap.add_argument("--labels", nargs="*", help="label files (label_pairs' ranges) that together cover the set")
+    ap.add_argument("--budget-s", type=float, help="cut the epochs to what fits this many seconds")
     a = ap.parse_args()
-    r = train(a.pairs, a.val, a.size, a.epochs, a.batch, a.lr, log=lambda s: print(s, flush=True), out=Path(a.out))
+    log = lambda s: print(s, flush=True)   # noqa: E731
+    total = a.pairs + a.val
+    if a.label_range:
+        lab = label_pairs(total, a.size, 1, a.label_range[0], a.label_range[1], a.grid, log)
+        np.savez(a.labels_out, **lab)
+        print(json.dumps({"labelled": int(len(lab["labels"])), "agreement": float(np.mean(lab["agree"]))}), flush=True)
+        return
+    labels = None
+    if a.labels:
+        parts = sorted((dict(np.load(f, allow_pickle=False)) for f in a.labels), key=lambda d: int(d["range"][0]))
+        labels = {k: np.concatenate([p[k] for p in parts]) for k in ("labels", "norms", "agree")}
+        labels["grid"] = parts[0]["grid"]
+    r = train(a.pairs, a.val, a.size, a.epochs, a.batch, a.lr, log=log, out=Path(a.out), labels=labels, grid=a.grid,
+              budget_s=a.budget_s)
     print(json.dumps(r), flush=True)
 
 
