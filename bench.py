@@ -447,11 +447,15 @@ def bench_prior(a, np, torch, zcmir_amd):
         A = np.asarray(full.H, float)[:2, :2]
         full_cell = prior.truth_cell(float(np.degrees(np.arctan2(A[1, 0] - A[0, 1], A[0, 0] + A[1, 1]))),
                                      float(np.sqrt(abs(np.linalg.det(A)))))
+        # ranks of the true and the full search's classes among all of the network's merged classes
+        order = prior.propose(net, mv[i][None], fx[i][None], k=prior.N_TH // 2 * prior.N_SG)[0]
+        rank = lambda c: order.index(c) // 2 if c in order else -1   # noqa: E731
         rows.append({"t_full_s": round(t_full, 4), "t_learned_s": round(t_lrn, 4), "pose_dist_px": round(d, 4),
                      "rot_diff_deg": round(dth, 4), "fell_back": bool(info["fell_back"]),
                      "check_score": round(info["check_score"], 4),
                      "truth_cell_proposed": prior.truth_cell(th[i], sg[i]) in info["cells"],
                      "full_cell_proposed": full_cell in info["cells"],
+                     "truth_class_rank": rank(prior.truth_cell(th[i], sg[i])), "full_class_rank": rank(full_cell),
                      "truth_dist_px": round(_pose_close(full.H, Hs[i], size, size)[0], 3),
                      "learned_truth_dist_px": round(_pose_close(lrn.H, Hs[i], size, size)[0], 3)})
     reg.close()
