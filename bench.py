@@ -152,7 +152,7 @@ def bench_map2048(a, np, torch, zcmir_amd, synth):
     reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=a.map2048_steps)
+    reg.bench_shift_map(np.eye(3), exact=True, calibrated=False, resolution=4096, iters=a.map2048_steps, warm=False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     reg.profile(True)
@@ -523,12 +523,13 @@ def main():
         raise SystemExit(f"bench workload check failed: peak ({sm.dx}, {sm.dy}), grid {sm.n}x{sm.ny}")
 
     # warmup, then exactly K maps bracketed by device synchronization (the engine's streams as
-    # in use: no per-launch events); then the same K maps again with a HIP event around every
-    # launch (one stream) for the per-kernel times of the roofline
+    # in use: no per-launch events, and no untimed map inside the window); then the same K maps
+    # again with a HIP event around every launch (one stream) for the per-kernel times of the
+    # roofline
     reg.bench_shift_map(H, exact=True, calibrated=False, resolution=2048, iters=a.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev_ms = reg.bench_shift_map(H, exact=True, calibrated=False, resolution=2048, iters=a.steps)
+    ev_ms = reg.bench_shift_map(H, exact=True, calibrated=False, resolution=2048, iters=a.steps, warm=False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0

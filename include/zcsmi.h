@@ -171,7 +171,8 @@ int32_t zc_fft_tiling(void* e, uint32_t L, uint32_t* cols);
 /* the rank transform of a side: rank (w*h f32, nullable), hist[257], cdf[258], stats[2] =
  * (mean, spread) — for bit-exact checks of the rank transform */
 int32_t zc_rank(void* e, uint32_t side, float* rank, size_t len, uint32_t* hist, uint32_t* cdf, float* stats);
-/* one untimed, unprofiled map, then `iters` shift maps back to back (flags, cap as
+/* one untimed, unprofiled map (left out with flags bit 4: the caller has already made one with
+ * the same settings), then `iters` shift maps back to back (flags bits 1 and 2, cap as
  * zc_shift_map); mean ms per map, HIP events */
 int32_t zc_bench_shift_map(void* e, const double H[9], uint32_t flags, uint32_t cap, uint32_t iters,
                            double* ms_per_map);

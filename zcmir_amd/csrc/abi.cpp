@@ -348,7 +348,7 @@ int32_t zc_bench_shift_map(void* p, const double* H, uint32_t flags, uint32_t ca
                            double* ms_per_map) {
   Engine* e = static_cast<Engine*>(p);
   return guard(e, "bench_shift_map", [&] {
-    {
+    if (!(flags & 4)) {
       // one untimed map first (buffers, tilings, twiddles in place), outside any profile
       struct Pause {
         zc::Prof& p;
