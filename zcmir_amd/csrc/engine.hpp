@@ -399,7 +399,7 @@ class Engine {
   DBuf sat_band, sat;   // rectangle plans: band sums and summed-area tables
   DBuf rs_fwd, rs_inv, rs_out, rs_tab;   // roto-scale maps: both directions, finished, log-polar tables
   DBuf pre_buf, pre_hist, pre_cdf, pre_part, pre_rgba;   // preprocessing scratch
-  DBuf sw_poses, sw_pose_fixed, sw_zero, sw_ns, sw_part, sw_peaks;
+  DBuf sw_poses, sw_pose_fixed, sw_zero, sw_dead, sw_ns, sw_part, sw_peaks;
   DBuf gn_jobs, gn_part, gn_sums;
   // the device Gauss-Newton batch (GnBuffers)
   DBuf gd_pairs, gd_state, gd_gjobs, gd_gsums, gd_tjobs, gd_tsums, gd_dir, gd_step, gd_try_H, gd_try_ok, gd_try_jobs,

@@ -45,9 +45,13 @@ static_assert(ColT<32, 16>::fits && ColT<64, 16>::fits && ColT<128, 16>::fits &&
                   ColT<2304, 4>::fits && ColT<3072, 4>::fits && ColT<4096, 4>::fits,
               "every default column tiling must be launchable");
 
+// skip (nullable; honoured for columns of up to 1024 points, where the check costs few
+// registers): per entry of `per` consecutive planes, output rows [x, y) that no later pass reads,
+// left unstored.
 template <int L, int S, int C>
 __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx, int planes, int rows_in, float scale,
-                                                          const float2* __restrict__ tw) {
+                                                          const float2* __restrict__ tw, const int4* __restrict__ skip,
+                                                          int per) {
   constexpr int T = ColT<L, C>::T, M = ColT<L, C>::M, STR = ColT<L, C>::STR;
   __shared__ float2 lds[C * STR];
   const int tid = threadIdx.x;
@@ -60,9 +64,16 @@ __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx
     return y < rows_in ? base[(unsigned)(y * nx + c)] : make_float2(0.0f, 0.0f);
   });
   rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
-  rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
-    base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
-  });
+  if constexpr (L <= 1024) {
+    const int4 sk = skip ? skip[plane / per] : make_int4(0, 0, 0, 0);
+    rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+      if (y < sk.x || y >= sk.y) base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
+    });
+  } else {
+    rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+      base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
+    });
+  }
 }
 
 // 2·C columns per workgroup in two groups of C sharing one LDS buffer (see rfft_groups).
@@ -207,26 +218,29 @@ static bool colfft_w_LC(float2* data, int planes, int nx, int rows_in, float sca
 }
 
 template <int L, int S, int C>
-static bool colfft_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st) {
+static bool colfft_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st,
+                      const int4* skip = nullptr, int per = 1) {
   if constexpr (!ColT<L, C>::fits) {
     return false;
   } else {
     if (nx % C) return false;
     const unsigned g = (unsigned)(nx / C * planes);
-    k_colfft<L, S, C><<<g, ColT<L, C>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw);
+    k_colfft<L, S, C><<<g, ColT<L, C>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw, skip, per);
     return true;
   }
 }
 
+// skip: output rows not to store (k_colfft), honoured by the plain tilings; the others store
+// every row
 template <int L, int S>
 static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, int lines,
-                     hipStream_t st) {
+                     hipStream_t st, const int4* skip = nullptr, int per = 1) {
   switch (lines > 0 ? lines : col_lines_default(L)) {
-    case 1: return colfft_LC<L, S, 1>(data, planes, nx, rows_in, scale, tw, st);
-    case 2: return colfft_LC<L, S, 2>(data, planes, nx, rows_in, scale, tw, st);
-    case 4: return colfft_LC<L, S, 4>(data, planes, nx, rows_in, scale, tw, st);
-    case 8: return colfft_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st);
-    case 16: return colfft_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st);
+    case 1: return colfft_LC<L, S, 1>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 2: return colfft_LC<L, S, 2>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 4: return colfft_LC<L, S, 4>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 8: return colfft_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 16: return colfft_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st, skip, per);
     // two groups sharing the LDS: 16 = 2 x 8, 32 = 2 x 16 columns
     case 17: return colfft_g2_LC<L, S, 8>(data, planes, nx, rows_in, scale, tw, st);
     case 33: return colfft_g2_LC<L, S, 16>(data, planes, nx, rows_in, scale, tw, st);
@@ -258,14 +272,14 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
 
 
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
-                   hipStream_t st, int lines) {
+                   hipStream_t st, int lines, const int4* skip, int per) {
   // a tiling that cannot launch at this width (nx not a multiple of its columns) falls back to
   // the default tiling
 #define CALL(LL)                                                                                   \
   {                                                                                                \
     auto run = [&](int ln) {                                                                       \
-      return inverse ? colfft_L<LL, 1>(data, planes, nx, rows_in, 1.0f, tw_y, ln, st)              \
-                     : colfft_L<LL, -1>(data, planes, nx, rows_in, 1.0f, tw_y, ln, st);            \
+      return inverse ? colfft_L<LL, 1>(data, planes, nx, rows_in, 1.0f, tw_y, ln, st, skip, per)   \
+                     : colfft_L<LL, -1>(data, planes, nx, rows_in, 1.0f, tw_y, ln, st, skip, per); \
     };                                                                                             \
     return run(lines) || (lines != 0 && run(0));                                                   \
   }

@@ -9,9 +9,11 @@ namespace zc {
 
 template <int L, int PID, int V>
 __global__ __launch_bounds__((RowX<L, V>::T)) void k_rowifft_extract(const float2* __restrict__ work, float* corr,
-                                                                    int ny, const float2* __restrict__ tw) {
+                                                                    int ny, const float2* __restrict__ tw,
+                                                                    const int4* __restrict__ dead) {
   constexpr CorrPlan P = kPlans[PID];
   constexpr int T = RowX<L, V>::T, M = RowX<L, V>::M, LINES = RowX<L, V>::LINES, STR = RowX<L, V>::STR;
+  constexpr int KN = P.kept_of[0];   // the plane of the overlap count n
   __shared__ float2 lds[LINES * STR];
   const int tid = threadIdx.x, g = blockIdx.y;
   const int q = g % P.n_out, b = g / P.n_out;
@@ -19,6 +21,17 @@ __global__ __launch_bounds__((RowX<L, V>::T)) void k_rowifft_extract(const float
   const int64_t N = (int64_t)L * ny;
   const float2* src = work + (int64_t)g * N;
   corr += (int64_t)b * P.n_corr * N;
+  const int4 d = dead ? dead[b] : make_int4(0, 0, 0, 0);
+  if (KN >= 0 && y0 >= d.x && y0 + LINES <= d.y) {
+    // every line of the workgroup is a lag row with no overlap: n = 0 there (the exact value of
+    // a correlation with nothing in common), and the score reads nothing else below its floor
+    if (q == KN / 2) {
+      float* on = corr + (int64_t)KN * N + (int64_t)y0 * L;
+      for (int i = tid; i < LINES * L; i += T)
+        if (y0 + i / L < ny) on[i] = 0.0f;
+    }
+    return;
+  }
   float2 v[M];
   rload<L, M, T, LINES, false>(v, tid, [&](int line, int x) {
     const int y = y0 + line;
@@ -31,8 +44,22 @@ __global__ __launch_bounds__((RowX<L, V>::T)) void k_rowifft_extract(const float
   rstore<L, M, T, LINES, false>(v, tid, [&](int line, int x, float2 val) {
     const int y = y0 + line;
     if (y < ny) {
-      o1[(int64_t)y * L + x] = val.x;
-      if (two) o2[(int64_t)y * L + x] = val.y;
+      if constexpr (L <= 1024 || LINES > 1) {
+        // short lines (few registers per lane) or several lines per workgroup: lag columns
+        // without overlap store n only; a lag row without overlap in a workgroup with other rows
+        // (the column pass may have left its input unstored) stores n = 0
+        const bool dead_row = y >= d.x && y < d.y;
+        const bool live = !dead_row && (x < d.z || x >= d.w);
+        if (2 * q == KN) o1[(int64_t)y * L + x] = dead_row ? 0.0f : val.x;
+        else if (live) o1[(int64_t)y * L + x] = val.x;
+        if (two) {
+          if (2 * q + 1 == KN) o2[(int64_t)y * L + x] = dead_row ? 0.0f : val.y;
+          else if (live) o2[(int64_t)y * L + x] = val.y;
+        }
+      } else {
+        o1[(int64_t)y * L + x] = val.x;
+        if (two) o2[(int64_t)y * L + x] = val.y;
+      }
     }
   });
 }
@@ -93,55 +120,57 @@ bool launch_rowifft_quads(int plan, const float2* work, float4* corr4, int nx, i
 }
 
 template <int L, int V>
-static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
+static void rowifft_LV(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch,
+                       const int4* dead) {
   using X = RowX<L, V>;
   dim3 g((ny + X::LINES - 1) / X::LINES, kPlans[plan].n_out * batch);
   switch (plan) {
-    case 0: k_rowifft_extract<L, 0, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 1: k_rowifft_extract<L, 1, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 2: k_rowifft_extract<L, 2, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 3: k_rowifft_extract<L, 3, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 4: k_rowifft_extract<L, 4, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 5: k_rowifft_extract<L, 5, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 6: k_rowifft_extract<L, 6, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 7: k_rowifft_extract<L, 7, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 8: k_rowifft_extract<L, 8, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 9: k_rowifft_extract<L, 9, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    case 10: k_rowifft_extract<L, 10, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-    default: k_rowifft_extract<L, 11, V><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+    case 0: k_rowifft_extract<L, 0, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 1: k_rowifft_extract<L, 1, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 2: k_rowifft_extract<L, 2, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 3: k_rowifft_extract<L, 3, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 4: k_rowifft_extract<L, 4, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 5: k_rowifft_extract<L, 5, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 6: k_rowifft_extract<L, 6, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 7: k_rowifft_extract<L, 7, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 8: k_rowifft_extract<L, 8, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 9: k_rowifft_extract<L, 9, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    case 10: k_rowifft_extract<L, 10, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+    default: k_rowifft_extract<L, 11, V><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
   }
 }
 
 template <int L>
-static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch) {
+static void rowifft_L(int plan, const float2* work, float* corr, int ny, const float2* tw, hipStream_t st, int batch,
+                      const int4* dead) {
   if constexpr (L % 3 == 0) {
     // lines of 2^a·3^b points (the sweep's planes): the default tiling, plans without rectangle sums
     using X = RowX<L, 0>;
     dim3 g((ny + X::LINES - 1) / X::LINES, kPlans[plan & 3].n_out * batch);
     switch (plan & 3) {
-      case 0: k_rowifft_extract<L, 0, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-      case 1: k_rowifft_extract<L, 1, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-      case 2: k_rowifft_extract<L, 2, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
-      default: k_rowifft_extract<L, 3, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw); break;
+      case 0: k_rowifft_extract<L, 0, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+      case 1: k_rowifft_extract<L, 1, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+      case 2: k_rowifft_extract<L, 2, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
+      default: k_rowifft_extract<L, 3, 0><<<g, X::T, 0, st>>>(work, corr, ny, tw, dead); break;
     }
     return;
   }
   if constexpr (L >= 256) {
     switch (FftRowTiling::v) {
-      case 1: rowifft_LV<L, 1>(plan, work, corr, ny, tw, st, batch); return;
-      case 2: rowifft_LV<L, 2>(plan, work, corr, ny, tw, st, batch); return;
-      case 3: rowifft_LV<L, 3>(plan, work, corr, ny, tw, st, batch); return;
-      case 4: rowifft_LV<L, 4>(plan, work, corr, ny, tw, st, batch); return;
+      case 1: rowifft_LV<L, 1>(plan, work, corr, ny, tw, st, batch, dead); return;
+      case 2: rowifft_LV<L, 2>(plan, work, corr, ny, tw, st, batch, dead); return;
+      case 3: rowifft_LV<L, 3>(plan, work, corr, ny, tw, st, batch, dead); return;
+      case 4: rowifft_LV<L, 4>(plan, work, corr, ny, tw, st, batch, dead); return;
       default: break;
     }
   }
-  rowifft_LV<L, 0>(plan, work, corr, ny, tw, st, batch);
+  rowifft_LV<L, 0>(plan, work, corr, ny, tw, st, batch, dead);
 }
 
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
-                            hipStream_t st, int batch) {
+                            hipStream_t st, int batch, const int4* dead) {
   if (nx % 3 == 0 && plan > 3) return false;
-#define CALL(LL) rowifft_L<LL>(plan, work, corr, ny, tw_x, st, batch)
+#define CALL(LL) rowifft_L<LL>(plan, work, corr, ny, tw_x, st, batch, dead)
   ZC_FFT_SWITCH(nx, CALL)
 #undef CALL
   return true;

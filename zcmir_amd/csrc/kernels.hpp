@@ -61,14 +61,24 @@ bool launch_pack_rowfft_shifted(int plan, int side, const float* stack, int sw, 
                                 int ch, int nx, int ny, float2* spec, const float2* tw_x, hipStream_t st);
 // column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
 // lines: columns per workgroup (0: the default for the length)
+// skip (nullable): per entry of `per` consecutive planes (x, y, ., .), output rows [x, y) that no
+// later pass reads; the plain tilings of columns of up to 1024 points leave them unstored
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
-                   hipStream_t st, int lines = 0);
+                   hipStream_t st, int lines = 0, const int4* skip = nullptr, int per = 1);
 // batch: spec_a and work hold `batch` consecutive entries (n_cplx / n_out planes each); spec_b
 // is shared
 void launch_cmul(int plan, const float2* spec_a, const float2* spec_b, float2* work, int nx, int ny,
                  hipStream_t st, int batch = 1);
+// dead (nullable): per batch entry (x, y, z, w), the lag rows [x, y) and lag columns [z, w) at
+// which the two windows' footprints do not overlap (every correlation is 0 there): a workgroup
+// whose rows all lie there stores n = 0 in the count plane and nothing else (n stays under any
+// overlap floor, so the score reads no other plane there); lines of up to 1024 points store
+// only n in those columns, and n = 0 in those rows of a workgroup that also holds other rows
+// (the column pass leaves such rows unstored at those lengths). Per-store checks cost the
+// longer lines' tilings registers and occupancy: they store every column and transform every
+// row of a workgroup that holds rows with overlap.
 bool launch_rowifft_extract(int plan, const float2* work, float* corr, int nx, int ny, const float2* tw_x,
-                            hipStream_t st, int batch = 1);
+                            hipStream_t st, int batch = 1, const int4* dead = nullptr);
 // Rectangle and translation plans (4..11), lines of 32..4096: the correlations as quads
 // (correlation k: component k & 3 of float4 plane k >> 2, (n_corr + 3) / 4 planes of nx · ny).
 bool launch_rowifft_quads(int plan, const float2* work, float4* corr4, int nx, int ny, const float2* tw_x,

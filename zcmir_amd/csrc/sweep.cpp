@@ -54,6 +54,7 @@ struct Cand {
   Mat3 H;
   uint32_t gx0, gy0, nx, ny;
   int taps;
+  uint32_t mw, mh;   // the moving footprint's cells from (gx0, gy0): window cells beyond are 0
 };
 
 }  // namespace
@@ -128,6 +129,8 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
            plane_len(mcx[1] - mcx[0] + Fx - 1, o.smooth), plane_len(mcy[1] - mcy[0] + Fy - 1, o.smooth), 1};
       // box-filter taps per cell side from the candidate's smallest stretch
       c.taps = (int)std::min(6.0, std::max(1.0, std::ceil(cell / (sv2(L)[1] * s0) - 1e-3)));
+      c.mw = mcx[1] - mcx[0];
+      c.mh = mcy[1] - mcy[0];
       if (c.nx > (uint32_t)kMaxFft || c.ny > (uint32_t)kMaxFft) throw std::runtime_error("FftTooLong");
     }
   }
@@ -140,6 +143,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   });
   std::vector<SweepPose> poses(nc);
   std::vector<int> zero(nc);
+  std::vector<int4> dead(nc);
   uint64_t nn_max = 0;
   for (uint32_t k = 0; k < nc; ++k) {
     const Cand& c = cands[perm[k]];
@@ -152,6 +156,9 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     const int64_t lx = (((int64_t)fcx[0] - c.gx0) % c.nx + c.nx) % c.nx;
     const int64_t ly = (((int64_t)fcy[0] - c.gy0) % c.ny + c.ny) % c.ny;
     zero[k] = (int)(ly * c.nx + lx);
+    // lags (moving index − fixed index) outside [−(F − 1), footprint − 1] pair the fixed window
+    // with the empty part of the moving one: plane rows (and columns) [footprint, plane − F + 1)
+    dead[k] = make_int4((int)c.mh, (int)(c.ny - Fy + 1), (int)c.mw, (int)(c.nx - Fx + 1));
     nn_max = std::max<uint64_t>(nn_max, (uint64_t)c.nx * c.ny);
   }
   SweepPose pf{};
@@ -191,6 +198,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   sw_poses.ensure(sizeof(SweepPose) * nc);
   sw_pose_fixed.ensure(sizeof(SweepPose));
   sw_zero.ensure(sizeof(int) * nc);
+  sw_dead.ensure(sizeof(int4) * nc);
   sw_peaks.ensure(sizeof(float2) * nc * n_peak);
   spec_b.ensure(nn_max * 8 * pi.n_cplx);
   spec_a.ensure(pts_max * 8 * pi.n_cplx);
@@ -203,6 +211,8 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   ZC_HIP(hipMemcpyAsync(sw_pose_fixed.p, &pf, sizeof pf, hipMemcpyHostToDevice, st));
   prof.break_chain();
   ZC_HIP(hipMemcpyAsync(sw_zero.p, zero.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
+  prof.break_chain();
+  ZC_HIP(hipMemcpyAsync(sw_dead.p, dead.data(), sizeof(int4) * nc, hipMemcpyHostToDevice, st));
   prof.break_chain();
   ensure_il(A);
   ensure_il(B);
@@ -240,17 +250,35 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     const int64_t nn = (int64_t)nx * ny;
     const float2* tx = twiddles((int)nx);
     const float2* ty = twiddles((int)ny);
-    SweepPack um{(float)ox, (float)oy, (float)cell, (int)grid, A.w, A.h, (int)(nx - Fx + 1), (int)(ny - Fy + 1), (int)ny};
+    // the window as wide and tall as the batch's largest footprint: the cells beyond it are 0 in
+    // every candidate's window, and the column FFT reads the rows not written as 0
+    uint32_t fw_b = 0, fh_b = 0;
+    for (int i = 0; i < b; ++i) {
+      fw_b = std::max(fw_b, cands[perm[c0 + i]].mw);
+      fh_b = std::max(fh_b, cands[perm[c0 + i]].mh);
+    }
+    SweepPack um{(float)ox, (float)oy, (float)cell, (int)grid, A.w, A.h, (int)std::min(nx - Fx + 1, fw_b),
+                 (int)std::min(ny - Fy + 1, fh_b), (int)ny};
     const double NCd = pi.n_cplx, Qd = pi.n_out, Cd = pi.n_corr, nnd = (double)nn;
     // algorithmic bytes of each launch: its role's minimal HBM traffic
     bytes("pack_sweep", b * (NCd * um.win_y * nx * 8 + 16.0 * A.w * A.h));
     bytes("colfft_fwd", b * NCd * (um.win_y * nx * 8.0 + nnd * 8));
     bytes("cmul", b * (2 * NCd + Qd) * nnd * 8);
     bytes("colfft_inv", b * 2 * Qd * nnd * 8);
-    bytes("rowifft_extract", b * (Qd * nnd * 8 + Cd * nnd * 4));
-    // every correlation plane at every lag (lags under the floor read only n, but their cache
-    // lines hold lags above it) and the score written
-    bytes("combine_sweep", b * (Cd + 1) * nnd * 4);
+    // the inverse rows: lag rows with overlap read Q outputs and store C planes (lines of up to
+    // 1024 points: n, and the other planes at the lag columns with overlap); rows without
+    // overlap store n = 0. The score: n and the score at every lag, the other planes at the lags
+    // with overlap (an upper bound of those above the floor).
+    double rows_b = 0, comb_b = 0;
+    for (int i = 0; i < b; ++i) {
+      const Cand& cd = cands[perm[c0 + i]];
+      const double ly = std::min<double>(ny, cd.mh + Fy - 1.0), lx = std::min<double>(nx, cd.mw + Fx - 1.0);
+      rows_b += nx <= 1024 ? ly * nx * (Qd * 8 + 4) + ly * lx * (Cd - 1) * 4 + (ny - ly) * nx * 4
+                           : ly * nx * (Qd * 8 + Cd * 4) + (ny - ly) * nx * 4;
+      comb_b += 2 * nnd * 4 + ly * lx * (Cd - 1) * 4;
+    }
+    bytes("rowifft_extract", rows_b);
+    bytes("combine_sweep", comb_b);
     bytes("peaks", b * nnd * 4);
     LAUNCH("pack_sweep", ok &= launch_pack_sweep(plan, A.il.as<float4>(), dposes + c0, b, um, (int)nx,
                                                   spec_a.as<float2>(), tx, st));
@@ -259,9 +287,9 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     LAUNCH("cmul", launch_cmul(plan, spec_a.as<float2>(), spec_b.as<float2>(), work.as<float2>(), (int)nx, (int)ny,
                                st, b));
     LAUNCH("colfft_inv", ok &= launch_colfft(work.as<float2>(), pi.n_out * b, (int)nx, (int)ny, (int)ny, true, ty,
-                                            st, col_lines((int)ny)));
+                                            st, col_lines((int)ny), sw_dead.as<int4>() + c0, pi.n_out));
     LAUNCH("rowifft_extract", ok &= launch_rowifft_extract(plan, work.as<float2>(), corr.as<float>(), (int)nx,
-                                                           (int)ny, tx, st, b));
+                                                           (int)ny, tx, st, b, sw_dead.as<int4>() + c0));
     LAUNCH("combine_sweep", launch_combine_sweep(plan, corr.as<float>(), sw_ns.as<float>(), nn, b, ca,
                                                  sw_zero.as<int>() + c0, st));
     LAUNCH("peaks", launch_peaks(sw_ns.as<float>(), (int)nx, (int)ny, b, (int)n_peak, sw_part.as<float2>(),
@@ -341,6 +369,10 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     const auto* p = reinterpret_cast<const uint8_t*>(&v);
     key.insert(key.end(), p, p + sizeof v);
   };
+  for (uint32_t k = 0; k < nc; ++k) {   // the footprints size each batch's pack window
+    put(cands[perm[k]].mw);
+    put(cands[perm[k]].mh);
+  }
   for (const Group& gr : groups) {
     put(gr);
     put(twiddles((int)gr.nx));
@@ -359,7 +391,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   put(ca);
   for (const int v : {A.w, A.h, B.w, B.h}) put(v);
   for (const void* p : {A.il.p, B.il.p, spec_a.p, spec_b.p, work.p, corr.p, sw_ns.p, sw_part.p, sw_peaks.p,
-                        sw_poses.p, sw_pose_fixed.p, sw_zero.p})
+                        sw_poses.p, sw_pose_fixed.p, sw_zero.p, sw_dead.p})
     put(p);
   run_captured(sweep_gc, std::move(key), prof.on || !set.b("sweep_graph"), issue);
   sweep_last_nx = groups.back().nx;
