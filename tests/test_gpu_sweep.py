@@ -190,3 +190,33 @@ def test_sweep_graph_replays_match_eager(reg_factory):
                 graph.shift_map(np.eye(3), exact=True, calibrated=False)   # eager launches between replays
     finally:
         lib.zc_sweep_tune(0)
+
+
+@pytest.mark.parametrize("rows", [1, 3, 4])
+def test_sweep_row_tilings_agree_on_lags_without_overlap(reg_factory, rows):
+    """A full-resolution candidate of a 640² pair (2048-point planes, about 400 lag rows and columns
+    where its footprint and the fixed image cannot overlap) gives the default tiling's map bit for
+    bit under the inverse row tilings that run the same radix stages at 2048 points (32 or 16
+    points per lane): tiling 4 holds two lines per workgroup, so lag rows without overlap next to
+    rows with it take the per-line path, while the others store n = 0 per whole workgroup. The
+    default tiling's candidate maps are checked against the oracle above."""
+    import zcmir_amd
+
+    lib = zcmir_amd._load()
+    mv, fx, _ = synth.rotscale_pair((640, 640), theta_deg=-37.0, scale=1 / 1.12, seed=9)
+    kw = dict(half=False, sw_full=True, sweep_smooth=False, **_single(37.0, 1.12))
+    out = {}
+    try:
+        for r in (0, rows):
+            lib.zc_fft_tune(r, 0)
+            reg = reg_factory(**kw)
+            reg.set_images(mv, fx, preprocess=False)
+            info, peaks, best = reg.sweep()
+            out[r] = (info["n"], reg.sweep_map(), peaks[0])
+    finally:
+        lib.zc_fft_tune(0, 0)
+    (n0, m0, p0), (n1, m1, p1) = out[0], out[rows]
+    assert n0 == n1 == 2048 and m0.shape == m1.shape == (2048, 2048)
+    assert (m0 == 0.0).mean() > 0.3, (m0 == 0.0).mean()   # lags below the floor, those without overlap among them
+    assert np.array_equal(m0, m1)
+    assert (p0["sx"], p0["sy"], p0["score"]) == (p1["sx"], p1["sy"], p1["score"])
