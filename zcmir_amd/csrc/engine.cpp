@@ -104,7 +104,7 @@ Engine::~Engine() {
   for (auto& s : sides) release_side(s);
   for (auto& s : stack) release_side(*s);
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
-                  &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
+                  &tmp_u32, &tmp_i64, &sw_poses, &sw_pose_fixed, &sw_zero, &sw_dead, &sw_ns, &sw_part, &sw_peaks, &gn_jobs, &gn_part,
                   &gn_sums, &sat_band, &sat, &rs_fwd, &rs_inv, &rs_out, &rs_tab, &pre_buf, &pre_hist, &pre_cdf,
                   &pre_part, &pre_rgba, &map_out, &gd_pairs, &gd_state, &gd_gjobs, &gd_gsums, &gd_tjobs, &gd_tsums,
                   &gd_dir, &gd_step, &gd_try_H, &gd_try_ok, &gd_try_jobs, &gd_try_sums, &gd_try_score, &gd_part, &gd_ctr,

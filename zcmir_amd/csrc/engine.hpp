@@ -41,6 +41,25 @@ namespace zc {
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
+  // a buffer an owner forgot to release is freed with it (the owner makes its device current)
+  ~DBuf() { release(); }
   // counts every allocation in the process: a captured graph whose key holds it is not replayed
   // after any buffer moved
   static inline std::atomic<uint64_t> gen{0};

@@ -45,9 +45,9 @@ static_assert(ColT<32, 16>::fits && ColT<64, 16>::fits && ColT<128, 16>::fits &&
                   ColT<2304, 4>::fits && ColT<3072, 4>::fits && ColT<4096, 4>::fits,
               "every default column tiling must be launchable");
 
-// skip (nullable; honoured for columns of up to 1024 points, where the check costs few
-// registers): per entry of `per` consecutive planes, output rows [x, y) that no later pass reads,
-// left unstored.
+// skip (nullable; honoured for columns of up to 1024 points and of 2^a·3^b points, where the
+// check costs no occupancy): per entry of `per` consecutive planes, output rows [x, y) that no
+// later pass reads, left unstored.
 template <int L, int S, int C>
 __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx, int planes, int rows_in, float scale,
                                                           const float2* __restrict__ tw, const int4* __restrict__ skip,
@@ -64,7 +64,7 @@ __global__ __launch_bounds__((ColT<L, C>::T)) void k_colfft(float2* data, int nx
     return y < rows_in ? base[(unsigned)(y * nx + c)] : make_float2(0.0f, 0.0f);
   });
   rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
-  if constexpr (L <= 1024) {
+  if constexpr (L <= 1024 || L % 3 == 0) {
     const int4 sk = skip ? skip[plane / per] : make_int4(0, 0, 0, 0);
     rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
       if (y < sk.x || y >= sk.y) base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
@@ -180,9 +180,12 @@ struct ColW {
       (M < 64 || T == 512);     // 64 points: 128 VGPRs of data, two waves per SIMD
 };
 
+// skip as in k_colfft, honoured for columns of 2^a·3^b points (the 2048 and 4096 tilings lose a
+// wave per SIMD to the check's registers)
 template <int L, int S, int C, int M, bool SPLIT>
 __global__ __launch_bounds__((ColW<L, C, M, SPLIT>::T), (SPLIT && M == 16 ? 2 : 1)) void k_colfft_w(float2* data, int nx, int planes, int rows_in,
-                                                                       float scale, const float2* __restrict__ tw) {
+                                                                       float scale, const float2* __restrict__ tw,
+                                                                       const int4* __restrict__ skip, int per) {
   constexpr int T = ColW<L, C, M, SPLIT>::T, STR = ColW<L, C, M, SPLIT>::STR;
   const int tid = threadIdx.x;
   const int groups = nx / C;
@@ -200,19 +203,28 @@ __global__ __launch_bounds__((ColW<L, C, M, SPLIT>::T), (SPLIT && M == 16 ? 2 : 
     __shared__ float2 lds[C * STR];
     rfft<L, M, T, C, true, S>(v, lds, STR, tw, tid);
   }
-  rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
-    base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
-  });
+  if constexpr (L % 3 == 0) {
+    const int4 sk = skip ? skip[plane / per] : make_int4(0, 0, 0, 0);
+    rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+      if (y < sk.x || y >= sk.y) base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
+    });
+  } else {
+    rstore<L, M, T, C, true>(v, tid, [&](int c, int y, float2 val) {
+      base[(unsigned)(y * nx + c)] = make_float2(val.x * scale, val.y * scale);
+    });
+  }
 }
 
 template <int L, int S, int C, int M, bool SPLIT>
-static bool colfft_w_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st) {
+static bool colfft_w_LC(float2* data, int planes, int nx, int rows_in, float scale, const float2* tw, hipStream_t st,
+                        const int4* skip = nullptr, int per = 1) {
   if constexpr (!ColW<L, C, M, SPLIT>::fits) {
     return false;
   } else {
     if (nx % C) return false;
     const unsigned g = (unsigned)(nx / C * planes);
-    k_colfft_w<L, S, C, M, SPLIT><<<g, ColW<L, C, M, SPLIT>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw);
+    k_colfft_w<L, S, C, M, SPLIT><<<g, ColW<L, C, M, SPLIT>::T, 0, st>>>(data, nx, planes, rows_in, scale, tw, skip,
+                                                                         per);
     return true;
   }
 }
@@ -252,20 +264,20 @@ static bool colfft_L(float2* data, int planes, int nx, int rows_in, float scale,
     // wide tilings (L >= 1024), 16 or 32 points per thread, split exchange unless noted:
     // 41: 4x16, 42: 8x16, 43: 8x32, 44: 16x32, 45: 8x32 plain, 46: 4x32, 47: 2x32 plain,
     // 48: 16x64, 49: 8x64
-    case 41: return colfft_w_LC<L, S, 4, 16, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 42: return colfft_w_LC<L, S, 8, 16, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 43: return colfft_w_LC<L, S, 8, 32, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 44: return colfft_w_LC<L, S, 16, 32, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 45: return colfft_w_LC<L, S, 8, 32, false>(data, planes, nx, rows_in, scale, tw, st);
-    case 46: return colfft_w_LC<L, S, 4, 32, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 47: return colfft_w_LC<L, S, 2, 32, false>(data, planes, nx, rows_in, scale, tw, st);
-    case 48: return colfft_w_LC<L, S, 16, 64, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 49: return colfft_w_LC<L, S, 8, 64, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 41: return colfft_w_LC<L, S, 4, 16, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 42: return colfft_w_LC<L, S, 8, 16, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 43: return colfft_w_LC<L, S, 8, 32, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 44: return colfft_w_LC<L, S, 16, 32, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 45: return colfft_w_LC<L, S, 8, 32, false>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 46: return colfft_w_LC<L, S, 4, 32, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 47: return colfft_w_LC<L, S, 2, 32, false>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 48: return colfft_w_LC<L, S, 16, 64, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 49: return colfft_w_LC<L, S, 8, 64, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
     // lines of 2^a·3^b points: 50: 8x48, 51: 4x48, 52: 8x24, 53: 4x24 (split exchange)
-    case 50: return colfft_w_LC<L, S, 8, 48, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 51: return colfft_w_LC<L, S, 4, 48, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 52: return colfft_w_LC<L, S, 8, 24, true>(data, planes, nx, rows_in, scale, tw, st);
-    case 53: return colfft_w_LC<L, S, 4, 24, true>(data, planes, nx, rows_in, scale, tw, st);
+    case 50: return colfft_w_LC<L, S, 8, 48, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 51: return colfft_w_LC<L, S, 4, 48, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 52: return colfft_w_LC<L, S, 8, 24, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
+    case 53: return colfft_w_LC<L, S, 4, 24, true>(data, planes, nx, rows_in, scale, tw, st, skip, per);
     default: return false;
   }
 }

@@ -62,7 +62,8 @@ bool launch_pack_rowfft_shifted(int plan, int side, const float* stack, int sw, 
 // column FFTs (length ny) of `planes` planes of nx columns, in place; rows >= rows_in read as 0
 // lines: columns per workgroup (0: the default for the length)
 // skip (nullable): per entry of `per` consecutive planes (x, y, ., .), output rows [x, y) that no
-// later pass reads; the plain tilings of columns of up to 1024 points leave them unstored
+// later pass reads; the plain tilings of columns of up to 1024 points and the plain and wide
+// tilings of 2^a·3^b points leave them unstored
 bool launch_colfft(float2* data, int planes, int nx, int ny, int rows_in, bool inverse, const float2* tw_y,
                    hipStream_t st, int lines = 0, const int4* skip = nullptr, int per = 1);
 // batch: spec_a and work hold `batch` consecutive entries (n_cplx / n_out planes each); spec_b

@@ -183,15 +183,18 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     uint32_t k0, k1, nx, ny, bn;
   };
   std::vector<Group> groups;
-  uint64_t part_max = 0;
+  uint64_t part_max = 0, pts_need = 0;
   for (uint32_t k0 = 0; k0 < nc;) {
     const uint32_t nx = cands[perm[k0]].nx, ny = cands[perm[k0]].ny;
     uint32_t k1 = k0;
     while (k1 < nc && cands[perm[k1]].nx == nx && cands[perm[k1]].ny == ny) ++k1;
     const uint64_t nn = (uint64_t)nx * ny;
-    const uint32_t bn = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.batch, pts_max / nn));
+    // no batch larger than its group: the buffers then hold what the sweep runs, not the budget
+    const uint32_t bn =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)o.batch, pts_max / nn, (uint64_t)(k1 - k0)}));
     groups.push_back({k0, k1, nx, ny, bn});
     part_max = std::max<uint64_t>(part_max, (uint64_t)bn * peaks_chunks((int64_t)nn));
+    pts_need = std::max<uint64_t>(pts_need, (uint64_t)bn * nn);
     k0 = k1;
   }
   const uint32_t n_peak = o.n_peak;
@@ -201,10 +204,10 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   sw_dead.ensure(sizeof(int4) * nc);
   sw_peaks.ensure(sizeof(float2) * nc * n_peak);
   spec_b.ensure(nn_max * 8 * pi.n_cplx);
-  spec_a.ensure(pts_max * 8 * pi.n_cplx);
-  work.ensure(pts_max * 8 * pi.n_out);
-  corr.ensure(pts_max * 4 * pi.n_corr);
-  sw_ns.ensure(pts_max * 4);
+  spec_a.ensure(pts_need * 8 * pi.n_cplx);
+  work.ensure(pts_need * 8 * pi.n_out);
+  corr.ensure(pts_need * 4 * pi.n_corr);
+  sw_ns.ensure(pts_need * 4);
   sw_part.ensure(sizeof(float2) * 256 * part_max);
   ZC_HIP(hipMemcpyAsync(sw_poses.p, poses.data(), sizeof(SweepPose) * nc, hipMemcpyHostToDevice, st));
   prof.break_chain();
