@@ -296,9 +296,13 @@ def _kernel_table(prof, pbytes, per):
     return out
 
 
-def _roofline(kernels, traffic=None):
+def _roofline(kernels, traffic=None, section=None):
+    """The dominant kernel's roofline; its PMC traffic per launch from profiles/hbm_traffic.json
+    `section` when given."""
     dom = max((k for k in kernels if kernels[k].get("gbs")), key=lambda k: kernels[k]["ms_per_unit"])
     d = kernels[dom]
+    if section is not None:
+        traffic = _traffic(section, dom)
     return {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
             "alg_bytes_per_launch": d["alg_bytes_per_unit"] / d["launches_per_unit"],
@@ -350,7 +354,7 @@ def bench_sweep(a, np, torch, zcmir_amd, synth):
             "candidates": info["nc"], "grid": info["grid"], "batch_tuning": tuning,
             "cell_px": round(info["cell"], 4), "largest_fft": info["n"],
             "fft_column_tiling": {str(info["n"]): reg.fft_tiling(info["n"])},
-            "roofline": _roofline(kernels),
+            "roofline": _roofline(kernels, section=f"sweep_{label}"),
             "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                         for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_unit"])},
         }

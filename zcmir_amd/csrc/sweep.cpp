@@ -189,12 +189,13 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     uint32_t k1 = k0;
     while (k1 < nc && cands[perm[k1]].nx == nx && cands[perm[k1]].ny == ny) ++k1;
     const uint64_t nn = (uint64_t)nx * ny;
-    // no batch larger than its group: the buffers then hold what the sweep runs, not the budget
-    const uint32_t bn =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)o.batch, pts_max / nn, (uint64_t)(k1 - k0)}));
+    const uint32_t bn = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.batch, pts_max / nn));
     groups.push_back({k0, k1, nx, ny, bn});
-    part_max = std::max<uint64_t>(part_max, (uint64_t)bn * peaks_chunks((int64_t)nn));
-    pts_need = std::max<uint64_t>(pts_need, (uint64_t)bn * nn);
+    // no batch runs larger than its group (nor larger than bn, tuned or not): the buffers hold
+    // what the sweep runs, not the budget
+    const uint64_t b_run = std::min<uint64_t>(bn, k1 - k0);
+    part_max = std::max<uint64_t>(part_max, b_run * peaks_chunks((int64_t)nn));
+    pts_need = std::max<uint64_t>(pts_need, b_run * nn);
     k0 = k1;
   }
   const uint32_t n_peak = o.n_peak;
@@ -267,19 +268,21 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     bytes("pack_sweep", b * (NCd * um.win_y * nx * 8 + 16.0 * A.w * A.h));
     bytes("colfft_fwd", b * NCd * (um.win_y * nx * 8.0 + nnd * 8));
     bytes("cmul", b * (2 * NCd + Qd) * nnd * 8);
-    bytes("colfft_inv", b * 2 * Qd * nnd * 8);
-    // the inverse rows: lag rows with overlap read Q outputs and store C planes (lines of up to
-    // 1024 points: n, and the other planes at the lag columns with overlap); rows without
-    // overlap store n = 0. The score: n and the score at every lag, the other planes at the lags
-    // with overlap (an upper bound of those above the floor).
-    double rows_b = 0, comb_b = 0;
+    // The inverse columns: Q planes read, their lag rows with overlap written (the rows without
+    // overlap are read by nothing). The inverse rows: lag rows with overlap read Q outputs and
+    // store C planes (lines of up to 1024 points: n, and the other planes at the lag columns with
+    // overlap); rows without overlap store n = 0. The score: n and the score at every lag, the
+    // other planes at the lags with overlap (an upper bound of those above the floor).
+    double cols_b = 0, rows_b = 0, comb_b = 0;
     for (int i = 0; i < b; ++i) {
       const Cand& cd = cands[perm[c0 + i]];
       const double ly = std::min<double>(ny, cd.mh + Fy - 1.0), lx = std::min<double>(nx, cd.mw + Fx - 1.0);
+      cols_b += Qd * nnd * 8 + Qd * ly * nx * 8;
       rows_b += nx <= 1024 ? ly * nx * (Qd * 8 + 4) + ly * lx * (Cd - 1) * 4 + (ny - ly) * nx * 4
                            : ly * nx * (Qd * 8 + Cd * 4) + (ny - ly) * nx * 4;
       comb_b += 2 * nnd * 4 + ly * lx * (Cd - 1) * 4;
     }
+    bytes("colfft_inv", cols_b);
     bytes("rowifft_extract", rows_b);
     bytes("combine_sweep", comb_b);
     bytes("peaks", b * nnd * 4);
