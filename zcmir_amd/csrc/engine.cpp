@@ -101,6 +101,10 @@ Engine::~Engine() {
   (void)hipSetDevice(dev);
   if (st) (void)hipStreamSynchronize(st);
   if (st2) (void)hipStreamSynchronize(st2);
+  // the captured graphs first: their launches name the buffers below
+  sweep_gc.reset();
+  gn_gc.reset();
+  climb_gc.reset();
   for (auto& s : sides) release_side(s);
   for (auto& s : stack) release_side(*s);
   for (DBuf* b : {&canvas_a, &canvas_b, &spec_a, &spec_b, &work, &corr, &map, &arg_part, &res, &parts, &sums,
@@ -110,9 +114,6 @@ Engine::~Engine() {
                   &gd_dir, &gd_step, &gd_try_H, &gd_try_ok, &gd_try_jobs, &gd_try_sums, &gd_try_score, &gd_part, &gd_ctr,
                   &cl_state, &cells_buf})
     b->release();
-  sweep_gc.reset();
-  gn_gc.reset();
-  climb_gc.reset();
   for (auto& kv : tw) kv.second.release();
   if (res_host) (void)hipHostFree(res_host);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
