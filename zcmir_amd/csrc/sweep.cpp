@@ -170,7 +170,8 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   // (every buffer scales with the batch's plane points). The budget: SweepTune::budget_gb, at
   // most half the device memory free now, at least 8 GiB (at 4096² planes a candidate's buffers
   // take 7.3 GB, and 4 per batch measured 6 % faster than 1).
-  uint64_t budget = (uint64_t)SweepTune::budget_gb.load() << 30;
+  const uint64_t budget_cfg = (uint64_t)SweepTune::budget_gb.load() << 30;
+  uint64_t budget = budget_cfg;
   {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
@@ -179,8 +180,11 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   auto per_cand = [&](uint64_t nn) { return nn * 8 * (pi.n_cplx + pi.n_out) + nn * 4 * (pi.n_corr + 1); };
   const uint64_t B_max = std::max<uint64_t>(1, std::min<uint64_t>(o.batch, budget / per_cand(nn_max)));
   const uint64_t pts_max = B_max * nn_max;   // plane points per batch
+  // bn: the batch bound now (the budget, capped by the free memory); bk: the bound of the
+  // configured budget alone, which keys the batch-size trials (the free memory changes with
+  // whatever else the process holds; the trials' result then applies up to bn)
   struct Group {
-    uint32_t k0, k1, nx, ny, bn;
+    uint32_t k0, k1, nx, ny, bn, bk;
   };
   std::vector<Group> groups;
   uint64_t part_max = 0, pts_need = 0;
@@ -190,7 +194,8 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
     while (k1 < nc && cands[perm[k1]].nx == nx && cands[perm[k1]].ny == ny) ++k1;
     const uint64_t nn = (uint64_t)nx * ny;
     const uint32_t bn = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.batch, pts_max / nn));
-    groups.push_back({k0, k1, nx, ny, bn});
+    const uint32_t bk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.batch, budget_cfg / per_cand(nn)));
+    groups.push_back({k0, k1, nx, ny, bn, bk});
     // no batch runs larger than its group (nor larger than bn, tuned or not): the buffers hold
     // what the sweep runs, not the budget
     const uint64_t b_run = std::min<uint64_t>(bn, k1 - k0);
@@ -312,7 +317,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   auto tuned_batch = [&](const Group& gr) -> uint32_t {
     static std::mutex mu;
     static std::map<std::array<uint32_t, 6>, uint32_t> best;
-    const std::array<uint32_t, 6> k{(uint32_t)plan, gr.nx, gr.ny, Fx, Fy, gr.bn};
+    const std::array<uint32_t, 6> k{(uint32_t)plan, gr.nx, gr.ny, Fx, Fy, gr.bk};
     {
       std::lock_guard<std::mutex> lk(mu);
       const auto it = best.find(k);
@@ -361,7 +366,7 @@ SweepOut Engine::sweep_sides(Side& A, Side& B, const SweepOpts& o) {
   };
   if (o.tune_batch && !prof.on)
     for (Group& gr : groups)
-      if (gr.k1 - gr.k0 > 1) gr.bn = tuned_batch(gr);
+      if (gr.k1 - gr.k0 > 1) gr.bn = std::min(gr.bn, tuned_batch(gr));
   auto issue = [&] {
     for (const Group& gr : groups) {
       fixed_batch(gr);
